@@ -1,0 +1,129 @@
+"""ctypes binding of libfwant.so (the C ABI declared in include/fwant.h).
+
+Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback in the product path.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfwant.so")
+
+FW_OK = 0
+NFIELD = 9
+FIELDS = ("Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
+
+# every symbol declared in include/fwant.h
+SYMBOLS = (
+    "fw_create", "fw_destroy", "fw_last_error", "fw_dims", "fw_set_grid", "fw_build_metric",
+    "fw_set_media", "fw_set_force", "fw_set_moment", "fw_set_receivers", "fw_add_snapshot",
+    "fw_init", "fw_get_pml_profiles", "fw_step", "fw_run", "fw_reference_steps", "fw_sync",
+    "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_get_wavefield",
+    "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
+    "fwk_stress2strain", "fwk_scatter", "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
+    "fwk_last_error",
+)
+
+
+class FwConfig(C.Structure):
+    _fields_ = [
+        ("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+        ("stept", C.c_float),
+        ("freenode", C.c_int),
+        ("abs_number", C.c_int * 6),
+        ("abs_velocity", C.c_float * 6),
+        ("cfs_bmax", C.c_float * 6),
+        ("cfs_amax", C.c_float * 6),
+    ]
+
+
+_lib = None
+
+fp = C.POINTER(C.c_float)
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+H = C.c_void_p
+
+
+def _sig(lib):
+    S = {
+        "fw_create": (C.c_int, [C.POINTER(FwConfig), C.POINTER(H)]),
+        "fw_destroy": (C.c_int, [H]),
+        "fw_last_error": (C.c_char_p, [H]),
+        "fw_dims": (C.c_int, [H, ip, ip, ip, ip]),
+        "fw_set_grid": (C.c_int, [H, fp, fp, fp, fp, fp, fp, fp, fp]),
+        "fw_build_metric": (C.c_int, [H, dp, dp, dp, fp, fp, fp, fp, fp]),
+        "fw_set_media": (C.c_int, [H, fp, fp, fp]),
+        "fw_set_force": (C.c_int, [H, C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, fp, fp, fp, fp]),
+        "fw_set_moment": (C.c_int, [H, C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, fp, fp]),
+        "fw_set_receivers": (C.c_int, [H, C.c_int, C.c_int, ip, C.c_int]),
+        "fw_add_snapshot": (C.c_int, [H, ip, ip, ip, C.c_int, C.c_uint, C.c_int, ip]),
+        "fw_init": (C.c_int, [H]),
+        "fw_get_pml_profiles": (C.c_int, [H, fp, fp, fp]),
+        "fw_step": (C.c_int, [H, C.c_int]),
+        "fw_run": (C.c_int, [H, C.c_int, C.c_int]),
+        "fw_reference_steps": (C.c_int, [C.c_int]),
+        "fw_sync": (C.c_int, [H]),
+        "fw_get_receivers": (C.c_int, [H, fp, ip]),
+        "fw_get_snapshot": (C.c_int, [H, C.c_int, fp, ip]),
+        "fw_snapshot_device": (C.c_int, [H, C.c_int, C.POINTER(vp), ip, C.POINTER(C.c_long)]),
+        "fw_get_wavefield": (C.c_int, [H, C.c_int, fp]),
+        "fw_set_wavefield": (C.c_int, [H, C.c_int, fp]),
+        "fw_enable_timing": (C.c_int, [H, C.c_int]),
+        "fw_get_timing": (C.c_int, [H, dp, dp, ip]),
+        "fwk_stress2strain": (C.c_int, [vp, C.c_long, C.c_long, C.c_long, C.c_int, vp, vp, vp]),
+        "fwk_scatter": (C.c_int, [vp, C.c_long, C.c_long, vp, C.c_long, C.c_long, C.c_long, C.c_int,
+                                  C.c_float, vp, vp, vp, vp, vp]),
+        "fwk_filter": (C.c_int, [C.c_int, C.c_long, C.c_int, C.c_int, dp, dp, vp, vp, vp]),
+        "fwk_scatter2kernel": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, fp, fp,
+                                         vp, vp, vp, vp, vp, vp, vp]),
+        "fwk_rotate": (C.c_int, [C.c_long, vp, vp, vp, fp, vp]),
+        "fwk_last_error": (C.c_char_p, []),
+    }
+    for name, (res, args) in S.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def lib():
+    """Return the loaded libfwant (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libfwant.so not found at {LIB_PATH}: build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'` (no CPU fallback exists)")
+        _lib = C.CDLL(LIB_PATH)
+        _sig(_lib)
+    return _lib
+
+
+class FwError(RuntimeError):
+    pass
+
+
+def check(rc, handle=None, what=""):
+    if rc != FW_OK:
+        L = lib()
+        msg = L.fw_last_error(handle) if handle is not None else L.fw_last_error(None)
+        raise FwError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def kcheck(rc, what=""):
+    if rc != FW_OK:
+        msg = lib().fwk_last_error()
+        raise FwError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+def fptr(a):
+    return a.ctypes.data_as(fp)
+
+
+def dptr(a):
+    return a.ctypes.data_as(dp)
+
+
+def iptr(a):
+    return a.ctypes.data_as(ip)

@@ -1,0 +1,128 @@
+// Shared device helpers for libfwant: DRP/opt MacCormack taps, source-time
+// functions, field indices.  gfx950 only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fw {
+
+enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TXY, TXZ, TYZ, NF };
+constexpr int LENFD = 3;
+
+// DRP/opt MacCormack 5-tap one-sided coefficients, code/srcF/mod_macdrp.h:19
+constexpr float MCA_1 = -0.30874f, MCA0 = -0.6326f, MCA1 = 1.233f, MCA2 = -0.3334f, MCA3 = 0.04168f;
+// 2-4 and 2-2 MacCormack operators, mod_macdrp.h:230-268
+constexpr float M24A0 = -7.0f / 6.0f, M24A1 = 8.0f / 6.0f, M24A2 = -1.0f / 6.0f;
+
+// One-sided operator along a stride s: forward uses taps -1..+3, backward
+// is the mirrored, negated stencil on taps -3..+1 (mod_macdrp.h:69-85).
+template <int FWD>
+__device__ __forceinline__ float dop(const float* __restrict__ p, long s) {
+  if constexpr (FWD)
+    return MCA_1 * p[-s] + MCA0 * p[0] + MCA1 * p[s] + MCA2 * p[2 * s] + MCA3 * p[3 * s];
+  else
+    return -MCA3 * p[-3 * s] - MCA2 * p[-2 * s] - MCA1 * p[-s] - MCA0 * p[0] - MCA_1 * p[s];
+}
+// Same operator on a 7-slot register window w[0..6] = offsets -3..+3.
+template <int FWD>
+__device__ __forceinline__ float dopw(const float* w) {
+  if constexpr (FWD)
+    return MCA_1 * w[2] + MCA0 * w[3] + MCA1 * w[4] + MCA2 * w[5] + MCA3 * w[6];
+  else
+    return -MCA3 * w[0] - MCA2 * w[1] - MCA1 * w[2] - MCA0 * w[3] - MCA_1 * w[4];
+}
+template <int FWD>
+__device__ __forceinline__ float d24w(const float* w) {
+  if constexpr (FWD)
+    return M24A0 * w[3] + M24A1 * w[4] + M24A2 * w[5];
+  else
+    return -M24A2 * w[1] - M24A1 * w[2] - M24A0 * w[3];
+}
+template <int FWD>
+__device__ __forceinline__ float d22w(const float* w) {
+  if constexpr (FWD)
+    return -w[3] + w[4];
+  else
+    return -w[2] + w[3];
+}
+
+// ---------------------------------------------------------------------------
+// Source-time functions, code/srcF/mod_src.F90:438-622.  Terms that involve
+// PI are formed in double (PI is double precision in the reference,
+// mod_constants.F90:12) and rounded to float.
+// ---------------------------------------------------------------------------
+enum {
+  SVF_BELL = 100, SVF_TRIANGLE = 110, SVF_GAUSS = 120, STF_RICKER = 200, STF_BSHIFT = 210,
+  STF_GAUSS = 220, STF_BELL = 230, STF_STEP = 240, STF_DELTA = 250
+};
+constexpr double PI_D = 3.141592653589793;
+constexpr float SEIS_ZERO = 1.0e-20f;
+
+__host__ __device__ inline float fun_gauss(float t, float a, float t0) {
+  if (fabsf(t0) >= SEIS_ZERO && (t <= 0.f || t >= 2.0f * t0)) return 0.f;
+  return (float)(exp((double)(-((t - t0) * (t - t0)) / (a * a))) / (sqrt(PI_D) * (double)a));
+}
+__host__ __device__ inline float fun_gauss_deriv(float t, float a, float t0) {
+  if (fabsf(t0) >= SEIS_ZERO && (t <= 0.f || t >= 2.0f * t0)) return 0.f;
+  return (float)(exp((double)(-((t - t0) * (t - t0)) / (a * a))) / (sqrt(PI_D) * (double)a) *
+                 (double)(-2.f * (t - t0) / (a * a)));
+}
+__host__ __device__ inline float fun_ricker(float t, float fc, float t0) {
+  if (t <= 0.f) return 0.f;
+  const float f0 = (float)(sqrt(PI_D) / 2.0);
+  const float u = (float)((double)((t - t0) * 2.0f) * PI_D * (double)fc);
+  return (u * u / 4.0f - 0.5f) * (float)exp((double)(-(u * u) / 4.0f)) * f0;
+}
+__host__ __device__ inline float fun_ricker_deriv(float t, float fc, float t0) {
+  if (t <= 0.f) return 0.f;
+  const float f0 = (float)(sqrt(PI_D) / 2.0);
+  const float u = (float)((double)((t - t0) * 2.0f) * PI_D * (double)fc);
+  return (float)((double)(u * (1.5f - u * u / 4.0f) * (float)exp((double)(-(u * u) / 4.0f)) * f0) * PI_D *
+                 (double)fc);
+}
+__host__ __device__ inline float fun_bell(float t, float r) {
+  return (t > 0.f && t < r) ? (float)((1.0 - cos(2 * PI_D * (double)t / (double)r)) / (double)r) : 0.f;
+}
+__host__ __device__ inline float fun_bell_deriv(float t, float r) {
+  return (t > 0.f && t < r) ? (float)(2.0 * PI_D * sin(2 * PI_D * (double)t / (double)r) / (double)r) : 0.f;
+}
+__host__ __device__ inline float fun_bell_int(float t, float r) {
+  if (t <= 0.f) return 0.f;
+  if (t < r) return (float)((double)(t / r) - sin(2 * PI_D * (double)t / (double)r) / (2.0 * PI_D));
+  return 1.f;
+}
+__host__ __device__ inline float fun_triangle(float t, float r) {
+  const float t0 = r / 2.0f;
+  if (t > r) return 0.f;
+  if (t > t0) return 2.0f / t0 - t / (t0 * t0);
+  if (t > 0.f) return t / (t0 * t0);
+  return 0.f;
+}
+// src_stf (which=0) / src_svf (which=1), mod_src.F90:475-533
+__host__ __device__ inline float stf_eval(int which, int id, float t, float t0, float f0, int ntime, float stept) {
+  if (which) {
+    float r;
+    switch (id) {
+      case SVF_GAUSS: r = fun_gauss(t, f0, t0); break;
+      case STF_GAUSS: r = fun_gauss_deriv(t, f0, t0); break;
+      case STF_RICKER: r = fun_ricker_deriv(t, f0, t0); break;
+      case SVF_BELL: r = fun_bell(t - t0, f0); break;
+      case STF_BELL: r = fun_bell_deriv(t - t0, f0); break;
+      case SVF_TRIANGLE: r = fun_triangle(t - t0, f0); break;
+      case STF_STEP: r = (ntime == 0) ? 1.0f / stept : 0.f; break;
+      default: r = 0.f;
+    }
+    return fabsf(r) < SEIS_ZERO ? 0.f : r;
+  }
+  switch (id) {
+    case STF_GAUSS: return fun_gauss(t, f0, t0);
+    case STF_RICKER: return fun_ricker(t, f0, t0);
+    case STF_BELL: return fun_bell(t - t0, f0);
+    case SVF_BELL: return fun_bell_int(t - t0, f0);
+    case STF_STEP: return 1.f;
+    case STF_DELTA: return (ntime == 0) ? 1.0f / stept : 0.f;
+    default: return 0.f;
+  }
+}
+
+}  // namespace fw

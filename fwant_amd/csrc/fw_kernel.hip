@@ -1,0 +1,295 @@
+// libfwant: sensitivity-kernel time integral of FWANT's SI_ker_sta
+// (code/srcF.kernel/SI_ker_sta.F90:1185-1557) on the GPU.
+//
+// Series layout: component c, sample n, point p at base[c*cs + n*ts + p];
+// threads run over points, so every access is coalesced.  Products are FP32,
+// accumulations FP64 (the compensated form of the reference's FP32 sums).
+#include "fw_common.h"
+#include "../../include/fwant.h"
+
+#include <string>
+#include <vector>
+
+namespace {
+thread_local std::string g_kerr;
+int kfail(int code, const std::string& m) {
+  g_kerr = m;
+  return code;
+}
+#define KCHK(expr)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return kfail(FW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// stress2strain, SI_ker_sta.F90:1185-1211
+__global__ void k_stress2strain(float* __restrict__ E, long cs, long ts, long npt, int knt,
+                                const float* __restrict__ lam, const float* __restrict__ mu) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  const float l = lam[p], u = mu[p];
+  const float E1 = (l + u) / (u * (3.0f * l + 2.0f * u));
+  const float E2 = -l / (2.0f * u * (3.0f * l + 2.0f * u));
+  const float E3 = 1.0f / u;
+  for (int n = blockIdx.y; n < knt; n += gridDim.y) {
+    float* e = E + (long)n * ts + p;
+    const float xx = e[0], yy = e[cs], zz = e[2 * cs];
+    const float E0 = E2 * (xx + yy + zz);
+    e[0] = E0 - (E2 - E1) * xx;
+    e[cs] = E0 - (E2 - E1) * yy;
+    e[2 * cs] = E0 - (E2 - E1) * zz;
+    e[3 * cs] = 0.5f * E3 * e[3 * cs];
+    e[4 * cs] = 0.5f * E3 * e[4 * cs];
+    e[5 * cs] = 0.5f * E3 * e[5 * cs];
+  }
+}
+
+// sgt2scatter3 for one receiver component (SI_ker_sta.F90:1290-1331).
+// Each thread owns one point and NB consecutive outputs n0..n0+NB-1 (1-based);
+// it walks m = 1 .. n0+NB-2 keeping a sliding register window of the source
+// strain S(n-m) for its NB outputs, so R(m) is loaded once per m.
+constexpr int NB = 8;
+__global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ R, long rcs, long rts,
+                                                 const float* __restrict__ S, long scs, long sts, long npt, int knt,
+                                                 float dt, const float* __restrict__ lam, const float* __restrict__ mu,
+                                                 float* __restrict__ outA, float* __restrict__ outB) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  // heavy tiles (large n0) first: blockIdx.y = 0 gets the last n block
+  const int nblk = (knt + NB - 1) / NB;
+  const int n0 = 1 + (nblk - 1 - (int)blockIdx.y) * NB;
+  float s[6][NB], tr[NB];
+  double a1[NB], a2[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    a1[q] = 0.0;
+    a2[q] = 0.0;
+    const int idx = n0 + q - 1;  // S index for m = 1
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s[c][q] = (idx >= 1 && idx <= knt) ? S[c * scs + (long)(idx - 1) * sts + p] : 0.f;
+    tr[q] = s[0][q] + s[1][q] + s[2][q];
+  }
+  const int mmax = min(n0 + NB - 2, knt - 1);
+  for (int m = 1; m <= mmax; ++m) {
+    const float* r = R + (long)(m - 1) * rts + p;
+    const float r0 = r[0], r1 = r[rcs], r2 = r[2 * rcs], r3 = r[3 * rcs], r4 = r[4 * rcs], r5 = r[5 * rcs];
+    const float trR = r0 + r1 + r2;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const float x1 = trR * tr[q];
+      const float x2 = r0 * s[0][q] + r1 * s[1][q] + r2 * s[2][q] +
+                       2.0f * (r3 * s[3][q] + r4 * s[4][q] + r5 * s[5][q]);
+      a1[q] += (double)x1;
+      a2[q] += (double)x2;
+    }
+    // slide the window: s[q] <- S(n0 + q - (m+1))
+#pragma unroll
+    for (int q = NB - 1; q > 0; --q) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) s[c][q] = s[c][q - 1];
+      tr[q] = tr[q - 1];
+    }
+    const int idx = n0 - m - 1;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s[c][0] = idx >= 1 ? S[c * scs + (long)(idx - 1) * sts + p] : 0.f;
+    tr[0] = s[0][0] + s[1][0] + s[2][0];
+  }
+  const double l = lam[p], u = mu[p];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int n = n0 + q;
+    if (n > knt) continue;
+    const double x1 = a1[q] * (double)dt, x2 = a2[q] * (double)dt;
+    outA[(long)(n - 1) * npt + p] = (float)(x1 * 2.0 * (l + 2.0 * u));
+    outB[(long)(n - 1) * npt + p] = (float)(2.0 * (x2 - x1) * 2.0 * u);
+  }
+}
+
+// IIR filter of SI_ker_sta.F90:1547-1557 over the padded series of
+// scatter_filter (mode 1) / scatter_filtfilt (mode 2), :1433-1475.
+constexpr int MAXNOD1 = 16;
+struct FiltCoef {
+  double b[MAXNOD1], a[MAXNOD1];
+};
+__device__ __forceinline__ double padded_x(const float* __restrict__ w0, long npt, long p, int n /*1-based*/, int knt,
+                                           int NFACT, int mode) {
+  if (n <= NFACT) {
+    if (mode == 1) return 0.0;
+    return 2.0 * (double)w0[p] - (double)w0[(long)(NFACT + 1 - n) * npt + p];
+  }
+  if (n <= NFACT + knt) return (double)w0[(long)(n - NFACT - 1) * npt + p];
+  const int q = n - NFACT - knt;  // 1..NFACT
+  return 2.0 * (double)w0[(long)(knt - 1) * npt + p] - (double)w0[(long)(knt - 1 - q) * npt + p];
+}
+__global__ __launch_bounds__(256) void k_filter(int mode, long npt, int knt, int nod1, FiltCoef cf,
+                                                const float* __restrict__ w0, float* __restrict__ w,
+                                                double* __restrict__ scratch) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  const int NOD = nod1 - 1, NFACT = 3 * NOD, NEL = knt + 2 * NFACT;
+  double xh[MAXNOD1], yh[MAXNOD1];  // xh[m] = x(n-m), yh[m] = y(n-m)
+#pragma unroll
+  for (int q = 0; q < MAXNOD1; ++q) { xh[q] = 0.0; yh[q] = 0.0; }
+  for (int n = 1; n <= NEL; ++n) {
+#pragma unroll
+    for (int q = MAXNOD1 - 1; q > 0; --q) xh[q] = xh[q - 1];
+    xh[0] = padded_x(w0, npt, p, n, knt, NFACT, mode);
+    double v = cf.b[0] * xh[0];
+#pragma unroll
+    for (int m = 2; m <= MAXNOD1; ++m)
+      if (m <= nod1 && m <= n) v = v + cf.b[m - 1] * xh[m - 1] - cf.a[m - 1] * yh[m - 2];
+    v = v / cf.a[0];
+#pragma unroll
+    for (int q = MAXNOD1 - 1; q > 0; --q) yh[q] = yh[q - 1];
+    yh[0] = v;
+    if (mode == 1) {
+      if (n > NFACT && n <= NFACT + knt) w[(long)(n - NFACT - 1) * npt + p] = (float)v;
+    } else {
+      scratch[(long)(n - 1) * npt + p] = v;
+    }
+  }
+  if (mode == 1) return;
+  // backward pass over the reversed forward output
+#pragma unroll
+  for (int q = 0; q < MAXNOD1; ++q) { xh[q] = 0.0; yh[q] = 0.0; }
+  for (int n = 1; n <= NEL; ++n) {
+#pragma unroll
+    for (int q = MAXNOD1 - 1; q > 0; --q) xh[q] = xh[q - 1];
+    xh[0] = scratch[(long)(NEL - n) * npt + p];
+    double v = cf.b[0] * xh[0];
+#pragma unroll
+    for (int m = 2; m <= MAXNOD1; ++m)
+      if (m <= nod1 && m <= n) v = v + cf.b[m - 1] * xh[m - 1] - cf.a[m - 1] * yh[m - 2];
+    v = v / cf.a[0];
+#pragma unroll
+    for (int q = MAXNOD1 - 1; q > 0; --q) yh[q] = yh[q - 1];
+    yh[0] = v;
+    // output sample NEL+1-n of the reversed result
+    const int o = NEL + 1 - n;
+    if (o > NFACT && o <= NFACT + knt) w[(long)(o - NFACT - 1) * npt + p] = (float)v;
+  }
+}
+
+// scatter2kernel, SI_ker_sta.F90:1375-1431 (trapezoid on [nt1, nt2])
+__global__ __launch_bounds__(256) void k_kernel(long npt, int nt1, int nt2, float dt, float V0, float U0,
+                                                const float* __restrict__ V, const float* __restrict__ U,
+                                                const float* __restrict__ Sa, const float* __restrict__ Sb,
+                                                float* __restrict__ Kap, float* __restrict__ Kaq,
+                                                float* __restrict__ Kbp, float* __restrict__ Kbq) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  auto sa = [&](int n) { return (double)Sa[(long)(n - 1) * npt + p]; };
+  auto sb = [&](int n) { return (double)Sb[(long)(n - 1) * npt + p]; };
+  const double v1 = V[nt1 - 1], v2 = V[nt2 - 1], u1 = U[nt1 - 1], u2 = U[nt2 - 1];
+  double kap = 0.5 * (v1 * sa(nt1) + v2 * sa(nt2));
+  double kaq = 0.5 * (u1 * sa(nt1) + u2 * sa(nt2));
+  double kbp = 0.5 * (v1 * sb(nt1) + v2 * sb(nt2));
+  double kbq = 0.5 * (u1 * sb(nt1) + u2 * sb(nt2));
+  for (int n = nt1 + 1; n <= nt2 - 1; ++n) {
+    const double a = sa(n), b = sb(n), v = V[n - 1], u = U[n - 1];
+    kap += v * a;
+    kaq += u * a;
+    kbp += v * b;
+    kbq += u * b;
+  }
+  Kap[p] = (float)(kap * (double)dt / (double)V0);
+  Kaq[p] = (float)(kaq * (double)dt / (double)U0);
+  Kbp[p] = (float)(kbp * (double)dt / (double)V0);
+  Kbq[p] = (float)(kbq * (double)dt / (double)U0);
+}
+
+struct Rot {
+  float r[9];
+};
+__global__ void k_rotate(long n, float* __restrict__ X, float* __restrict__ Y, float* __restrict__ Z, Rot R) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const float v1 = X[q], v2 = Y[q], v3 = Z[q];
+  X[q] = v1 * R.r[0] + v2 * R.r[1] + v3 * R.r[2];
+  Y[q] = v1 * R.r[3] + v2 * R.r[4] + v3 * R.r[5];
+  Z[q] = v1 * R.r[6] + v2 * R.r[7] + v3 * R.r[8];
+}
+
+unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+}  // namespace
+
+extern "C" {
+
+const char* fwk_last_error(void) { return g_kerr.c_str(); }
+
+int fwk_stress2strain(float* E, long cs, long ts, long npt, int knt, const float* lam, const float* mu, void* stream) {
+  if (!E || !lam || !mu || npt <= 0 || knt <= 0) return kfail(FW_ERR_ARG, "fwk_stress2strain: bad argument");
+  const unsigned gy = (unsigned)std::min(knt, 64);
+  hipLaunchKernelGGL(k_stress2strain, dim3(nblocks(npt, 256), gy), dim3(256), 0, (hipStream_t)stream, E, cs, ts, npt,
+                     knt, lam, mu);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
+
+int fwk_scatter(const float* R, long rcs, long rts, const float* S, long scs, long sts, long npt, int knt, float dt,
+                const float* lam, const float* mu, float* outA, float* outB, void* stream) {
+  if (!R || !S || !lam || !mu || !outA || !outB || npt <= 0 || knt <= 0)
+    return kfail(FW_ERR_ARG, "fwk_scatter: bad argument");
+  const unsigned gy = (unsigned)((knt + NB - 1) / NB);
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(npt, 256), gy), dim3(256), 0, (hipStream_t)stream, R, rcs, rts, S, scs,
+                     sts, npt, knt, dt, lam, mu, outA, outB);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
+
+int fwk_filter(int mode, long npt, int knt, int nod1, const double* b, const double* a, const float* w0, float* w,
+               void* stream) {
+  if ((mode != 1 && mode != 2) || npt <= 0 || knt <= 0 || nod1 < 1 || nod1 > MAXNOD1 || !b || !a || !w0 || !w)
+    return kfail(FW_ERR_ARG, "fwk_filter: bad argument (mode 1|2, 1 <= nod1 <= 16)");
+  if (knt <= 3 * (nod1 - 1)) return kfail(FW_ERR_ARG, "fwk_filter: knt must exceed 3*(nod1-1)");
+  if (a[0] == 0.0) return kfail(FW_ERR_ARG, "fwk_filter: a(1) must be nonzero");
+  FiltCoef cf{};
+  for (int q = 0; q < nod1; ++q) { cf.b[q] = b[q]; cf.a[q] = a[q]; }
+  const int NEL = knt + 6 * (nod1 - 1);
+  double* scratch = nullptr;
+  hipStream_t st = (hipStream_t)stream;
+  // filtfilt keeps the whole forward-pass output (NEL doubles per point)
+  if (mode == 2) KCHK(hipMallocAsync((void**)&scratch, sizeof(double) * (size_t)NEL * npt, st));
+  hipLaunchKernelGGL(k_filter, dim3(nblocks(npt, 256)), dim3(256), 0, st, mode, npt, knt, nod1, cf, w0, w, scratch);
+  KCHK(hipGetLastError());
+  if (scratch) KCHK(hipFreeAsync(scratch, st));
+  return FW_OK;
+}
+
+int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, const float* V, const float* U,
+                       const float* Sa, const float* Sb, float* Kap, float* Kaq, float* Kbp, float* Kbq, void* stream) {
+  if (npt <= 0 || knt <= 0 || nt1 < 1 || nt2 > knt || nt1 >= nt2 || !V || !U || !Sa || !Sb || !Kap || !Kaq || !Kbp ||
+      !Kbq)
+    return kfail(FW_ERR_ARG, "fwk_scatter2kernel: bad argument (1 <= nt1 < nt2 <= knt)");
+  // normalisation of SI_ker_sta.F90:1391-1397 (host, double)
+  double V0 = (0.5 * (double)V[nt1 - 1] * V[nt1 - 1] + 0.5 * (double)V[nt2 - 1] * V[nt2 - 1]) * dt;
+  double U0 = (0.5 * (double)U[nt1 - 1] * U[nt1 - 1] + 0.5 * (double)U[nt2 - 1] * U[nt2 - 1]) * dt;
+  for (int n = nt1 + 1; n <= nt2 - 1; ++n) {
+    V0 += (double)V[n - 1] * V[n - 1] * dt;
+    U0 += (double)U[n - 1] * U[n - 1] * dt;
+  }
+  V0 *= M0;
+  U0 *= M0;
+  hipStream_t st = (hipStream_t)stream;
+  float* dVU = nullptr;
+  KCHK(hipMallocAsync((void**)&dVU, sizeof(float) * 2 * knt, st));
+  KCHK(hipMemcpyAsync(dVU, V, sizeof(float) * knt, hipMemcpyHostToDevice, st));
+  KCHK(hipMemcpyAsync(dVU + knt, U, sizeof(float) * knt, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_kernel, dim3(nblocks(npt, 256)), dim3(256), 0, st, npt, nt1, nt2, dt, (float)V0, (float)U0, dVU,
+                     dVU + knt, Sa, Sb, Kap, Kaq, Kbp, Kbq);
+  KCHK(hipGetLastError());
+  KCHK(hipFreeAsync(dVU, st));
+  KCHK(hipStreamSynchronize(st));
+  return FW_OK;
+}
+
+int fwk_rotate(long n, float* X, float* Y, float* Z, const float* rot, void* stream) {
+  if (n <= 0 || !X || !Y || !Z || !rot) return kfail(FW_ERR_ARG, "fwk_rotate: bad argument");
+  Rot R;
+  for (int q = 0; q < 9; ++q) R.r[q] = rot[q];
+  hipLaunchKernelGGL(k_rotate, dim3(nblocks(n, 256)), dim3(256), 0, (hipStream_t)stream, n, X, Y, Z, R);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
+
+}  // extern "C"

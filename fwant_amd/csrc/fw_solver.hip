@@ -1,0 +1,1447 @@
+// libfwant: MI355X-native velocity-stress solver of FWANT's seis3d_wave
+// (DRP/opt MacCormack + RK4 + CFS-PML + free-surface imaging + sources).
+//
+// Device layout (per handle):
+//   4 wavefield register sets A, B, C, T; each is 9 SoA fields of a pitched
+//   column-major ghosted volume (i fastest, row pitch px = nx rounded up to
+//   16 floats, interior column ni1 aligned to 64 bytes).
+//   A holds W_n (and doubles as the RK "m" register); B and C ping-pong the
+//   stage states; T is the RK accumulator.  Per step:
+//     stage 1: A -> B, T = A + b1 h      stage 2: B -> C (m = A), T += b2 h
+//     stage 3: C -> B (m = A), T += b3 h stage 4: B -> A = T + b4 h
+//   which is the minimal-traffic form of macdrp_syn/RK_beg/inn/fin
+//   (code/srcF/mod_macdrp.F90:268-393).
+//   PML auxiliary variables use the same A/B/C/T rotation per face slab.
+//
+// Kernels per RK stage:
+//   k_surface   free-surface stress imaging (mod_macdrp.F90:3414-3436),
+//               saving the pre-image surface stresses at stage 1
+//   k_stage     fused operator sweep + RK update (mod_macdrp.F90:790-962,
+//               291-393); z-marching columns, register window in z
+//   k_pml       CFS-PML correction of one face slab, applied as an RK
+//               fix-up on the stage outputs, plus the auxiliary RK update
+//               (mod_abs_npml.F90:1149-2123, 738-1105)
+//   k_src       point-source terms (mod_src.F90:207-414), RK fix-up
+#include "fw_common.h"
+#include "../../include/fwant.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace fw;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      return fail(h, FW_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    }                                                                             \
+  } while (0)
+
+constexpr int BX = 64, BY = 4;
+
+struct StageArgs {
+  const float* __restrict__ win;
+  float* __restrict__ wout;
+  const float* __restrict__ m;
+  float* __restrict__ t;
+  const float* __restrict__ lam;
+  const float* __restrict__ mu;
+  const float* __restrict__ rrho;
+  const float* __restrict__ xi_x;
+  const float* __restrict__ eta_y;
+  const float* __restrict__ zeta_z;
+  const float* __restrict__ rz;
+  const float* __restrict__ rxsin;
+  const float* __restrict__ xcot;
+  const float* __restrict__ msurf;  // saved Tzz, Txz, Tyz at k = nk2 (3 planes)
+  long fs, sy, sz, ps;               // field stride, j stride, k stride, plane-array stride
+  int ni1, ni2, nj1, nj2, nk1, nk2;
+  int kchunk, freenode;
+  float a, b;
+};
+
+// ---------------------------------------------------------------------------
+// Fused stage kernel: RHS of Lx?_Ly?_Lz? (mod_macdrp.F90:790-962) for one
+// (i, j) column per thread, marching k with a 7-slot register window for the
+// z-derivatives of Tzz Txz Tyz Vx Vy Vz, followed by the RK update.
+//   KIND 0: Wout = m + a h, T = m + b h (m = Win)     RK_beg
+//   KIND 1: Wout = m + a h, T = T + b h               RK_inn
+//   KIND 2: Wout = T + b h                            RK_fin
+// The free-surface traction condition at k = nk2 is applied with zero source
+// terms; k_src adds the source traction-rate part.
+// ---------------------------------------------------------------------------
+template <int XF, int YF, int ZF, int KIND>
+__global__ __launch_bounds__(BX* BY) void k_stage(StageArgs g) {
+  const int i = g.ni1 + blockIdx.x * BX + threadIdx.x;
+  const int j = g.nj1 + blockIdx.y * BY + threadIdx.y;
+  if (i > g.ni2 || j > g.nj2) return;
+  const int k0 = g.nk1 + blockIdx.z * g.kchunk;
+  const int k1 = min(k0 + g.kchunk - 1, g.nk2);
+  const long fs = g.fs, sy = g.sy, sz = g.sz;
+  const float xi = g.xi_x[i], et = g.eta_y[j], rxs = g.rxsin[i], xc = g.xcot[i];
+  const float* __restrict__ W = g.win;
+
+  constexpr int ZFLD[6] = {TZZ, TXZ, TYZ, VX, VY, VZ};
+  float w[6][7];
+  long c = i + sy * j + sz * (long)k0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+#pragma unroll
+    for (int o = 0; o < 7; ++o) w[q][o] = 0.f;
+#pragma unroll
+    for (int o = (ZF ? -1 : -3); o <= (ZF ? 2 : 0); ++o) w[q][o + 3] = W[ZFLD[q] * fs + c + o * sz];
+  }
+
+  for (int k = k0; k <= k1; ++k, c += sz) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if constexpr (ZF)
+        w[q][6] = W[ZFLD[q] * fs + c + 3 * sz];
+      else
+        w[q][4] = W[ZFLD[q] * fs + c + sz];
+    }
+    const float lam = g.lam[c], miu = g.mu[c], rr = g.rrho[c];
+    const float lam2mu = lam + 2.0f * miu;
+    const float ze = g.zeta_z[k], rzk = g.rz[k];
+
+    const float* pTxx = W + TXX * fs + c;
+    const float* pTyy = W + TYY * fs + c;
+    const float* pTxy = W + TXY * fs + c;
+    const float* pTxz = W + TXZ * fs + c;
+    const float* pTyz = W + TYZ * fs + c;
+    const float* pVx = W + VX * fs + c;
+    const float* pVy = W + VY * fs + c;
+    const float* pVz = W + VZ * fs + c;
+
+    float DxTxx = dop<XF>(pTxx, 1) * xi;
+    float DxTxy = dop<XF>(pTxy, 1) * xi;
+    float DxTxz = dop<XF>(pTxz, 1) * xi;
+    const float DxVx = dop<XF>(pVx, 1) * xi;
+    const float DxVy = dop<XF>(pVy, 1) * xi;
+    const float DxVz = dop<XF>(pVz, 1) * xi;
+    const float DyTyy = dop<YF>(pTyy, sy) * et;
+    float DyTxy = dop<YF>(pTxy, sy) * et;
+    float DyTyz = dop<YF>(pTyz, sy) * et;
+    const float DyVx = dop<YF>(pVx, sy) * et;
+    const float DyVy = dop<YF>(pVy, sy) * et;
+    const float DyVz = dop<YF>(pVz, sy) * et;
+    const float DzTzz = dopw<ZF>(w[0]) * ze;
+    float DzTxz = dopw<ZF>(w[1]) * ze;
+    float DzTyz = dopw<ZF>(w[2]) * ze;
+    float DzVx, DzVy, DzVz;
+    if (g.freenode && k == g.nk2 - 1) {  // CondFreeVLOW, mod_macdrp.F90:872-886
+      DzVx = d22w<ZF>(w[3]) * ze;
+      DzVy = d22w<ZF>(w[4]) * ze;
+      DzVz = d22w<ZF>(w[5]) * ze;
+    } else if (g.freenode && k == g.nk2 - 2) {  // mod_macdrp.F90:887-900
+      DzVx = d24w<ZF>(w[3]) * ze;
+      DzVy = d24w<ZF>(w[4]) * ze;
+      DzVz = d24w<ZF>(w[5]) * ze;
+    } else {
+      DzVx = dopw<ZF>(w[3]) * ze;
+      DzVy = dopw<ZF>(w[4]) * ze;
+      DzVz = dopw<ZF>(w[5]) * ze;
+    }
+    if (miu <= SEIS_ZERO) {  // fluid cells, mod_macdrp.F90:917-926
+      DxTxy = 0.f; DxTxz = 0.f; DyTxy = 0.f; DyTyz = 0.f; DzTxz = 0.f; DzTyz = 0.f;
+    }
+    const float Txx = pTxx[0], Tyy = pTyy[0], Txy = pTxy[0];
+    const float Tzz = w[0][3], Txz = w[1][3], Tyz = w[2][3];
+    const float Vx = w[3][3], Vy = w[4][3], Vz = w[5][3];
+
+    float h[9];
+    // momentum with the spherical geometric terms, mod_macdrp.F90:928-933
+    h[VX] = rr * (DxTxx * rzk + DyTxy * rzk * rxs + DzTxz + (3.0f * Txz + Txx * xc - Tyy * xc) * rzk);
+    h[VY] = rr * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
+    h[VZ] = rr * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
+    // strain rates, mod_macdrp.F90:935-941
+    const float E11 = (DxVx + Vz) * rzk;
+    const float E22 = (Vx * xc + DyVy * rxs + Vz) * rzk;
+    const float E12 = (DxVy + DyVx * rxs - Vy * xc) * rzk * 0.5f;
+    float E33 = DzVz;
+    float E13 = (DxVz * rzk + DzVx - Vx * rzk) * 0.5f;
+    float E23 = (DyVz * rzk * rxs + DzVy - Vy * rzk) * 0.5f;
+    const bool surf = g.freenode && k == g.nk2;
+    if (surf) {  // traction condition, mod_macdrp.F90:942-950 (source part in k_src)
+      E33 = -lam / lam2mu * (E11 + E22);
+      E13 = 0.f;
+      E23 = 0.f;
+    }
+    // Hooke, mod_macdrp.F90:952-957
+    h[TXX] = lam2mu * E11 + lam * (E22 + E33);
+    h[TYY] = lam2mu * E22 + lam * (E11 + E33);
+    h[TZZ] = lam2mu * E33 + lam * (E11 + E22);
+    h[TXY] = 2.0f * miu * E12;
+    h[TXZ] = 2.0f * miu * E13;
+    h[TYZ] = 2.0f * miu * E23;
+
+    if constexpr (KIND == 2) {
+#pragma unroll
+      for (int f = 0; f < 9; ++f) g.wout[f * fs + c] = g.t[f * fs + c] + g.b * h[f];
+    } else {
+      float mc[9];
+      if constexpr (KIND == 0) {
+        mc[VX] = Vx; mc[VY] = Vy; mc[VZ] = Vz; mc[TXX] = Txx; mc[TYY] = Tyy; mc[TXY] = Txy;
+        mc[TZZ] = Tzz; mc[TXZ] = Txz; mc[TYZ] = Tyz;
+      } else {
+#pragma unroll
+        for (int f = 0; f < 9; ++f) mc[f] = g.m[f * fs + c];
+      }
+      if (surf) {  // pre-image surface stresses are the RK "m" values there
+        const long c2 = i + sy * j;
+        mc[TZZ] = g.msurf[c2];
+        mc[TXZ] = g.msurf[g.ps + c2];
+        mc[TYZ] = g.msurf[2 * g.ps + c2];
+      }
+#pragma unroll
+      for (int f = 0; f < 9; ++f) {
+        g.wout[f * fs + c] = mc[f] + g.a * h[f];
+        if constexpr (KIND == 0)
+          g.t[f * fs + c] = mc[f] + g.b * h[f];
+        else
+          g.t[f * fs + c] = g.t[f * fs + c] + g.b * h[f];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+#pragma unroll
+      for (int o = 0; o < 6; ++o) w[q][o] = w[q][o + 1];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Free-surface stress imaging (free_timg, mod_macdrp.F90:3414-3436) on the
+// stage input, with the surface traction of surface force sources
+// (src_surface, mod_src.F90:373-414).  At stage 1 the pre-image surface
+// stresses are saved first: they are the RK "m" values at k = nk2.
+// ---------------------------------------------------------------------------
+struct SurfEntry {  // one surface force point (the last (n, m) writing it wins)
+  int n, m;
+  float rS;  // unused placeholder for alignment
+  float S;   // area (x(i+1)-x(i-1))(y(j+1)-y(j-1))/4 z^2 sin
+};
+struct SurfArgs {
+  float* win;
+  float* msurf;
+  const int* __restrict__ smap;  // plane map -> entry or -1
+  const SurfEntry* __restrict__ ent;
+  const float* __restrict__ ft0;    // force start time [n]
+  const float* __restrict__ ftime;  // window t0 [ntw]
+  const float* __restrict__ ffreq;  // window f0 [ntw]
+  const float* __restrict__ F;      // Fx Fy Fz blocks [3][ntw*n]
+  long fs, sy, sz, ps, fblk;
+  int ni1, ni2, nj1, nj2, nk2, ntw, stf_id, ntime, save;
+  float ts, stept;
+};
+__global__ void k_surface(SurfArgs g) {
+  const int i = g.ni1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = g.nj1 + blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > g.ni2 || j > g.nj2) return;
+  const long c2 = i + g.sy * j;
+  const long cs = c2 + g.sz * g.nk2;
+  float* Tzz = g.win + TZZ * g.fs;
+  float* Txz = g.win + TXZ * g.fs;
+  float* Tyz = g.win + TYZ * g.fs;
+  if (g.save) {
+    g.msurf[c2] = Tzz[cs];
+    g.msurf[g.ps + c2] = Txz[cs];
+    g.msurf[2 * g.ps + c2] = Tyz[cs];
+  }
+  float tx = 0.f, ty = 0.f, tz = 0.f;
+  const int e = g.smap[c2];
+  if (e >= 0) {
+    const SurfEntry se = g.ent[e];
+    const float disp = stf_eval(0, g.stf_id, g.ts - g.ft0[se.n], g.ftime[se.m], g.ffreq[se.m], g.ntime, g.stept);
+    const long q = se.m + (long)g.ntw * se.n;
+    tx = g.F[q] / se.S * disp;
+    ty = g.F[g.fblk + q] / se.S * disp;
+    tz = g.F[2 * g.fblk + q] / se.S * disp;
+  }
+#pragma unroll
+  for (int k = 1; k <= 3; ++k) {
+    Tzz[cs + k * g.sz] = 2.0f * tz - Tzz[cs - k * g.sz];
+    Txz[cs + k * g.sz] = 2.0f * tx - Txz[cs - k * g.sz];
+    Tyz[cs + k * g.sz] = 2.0f * ty - Tyz[cs - k * g.sz];
+  }
+  Tzz[cs] = tz;
+  Txz[cs] = tx;
+  Tyz[cs] = ty;
+}
+
+// ---------------------------------------------------------------------------
+// CFS-PML correction on one face slab (abs_mod Lx?_Ly?_Lz?, mod_abs_npml.F90
+// :1149-2123) applied as an RK fix-up of the stage outputs written by k_stage
+// (RK is linear in h), plus the auxiliary-variable RK update
+// (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105).
+// Auxiliary variable order per face: x: Vx Vy Vz Txx Txy Txz;
+// y: Vx Vy Vz Tyy Txy Tyz; z: Vx Vy Vz Tzz Txz Tyz.
+// ---------------------------------------------------------------------------
+struct PmlArgs {
+  const float* __restrict__ win;
+  float* __restrict__ wout;
+  float* __restrict__ t;
+  const float* __restrict__ lam;
+  const float* __restrict__ mu;
+  const float* __restrict__ rrho;
+  const float* __restrict__ xi_x;
+  const float* __restrict__ eta_y;
+  const float* __restrict__ zeta_z;
+  const float* __restrict__ rz;
+  const float* __restrict__ rxsin;
+  const float* __restrict__ xcot;
+  const float* __restrict__ pb1;  // 1/B along the face axis (ghosted index)
+  const float* __restrict__ pad;  // A + D/B
+  const float* __restrict__ pdd;  // D/B
+  const float* __restrict__ pin;
+  float* __restrict__ pout;
+  const float* __restrict__ pm;
+  float* __restrict__ pt;
+  long fs, sy, sz, pcnt;
+  int lo0, lo1, lo2, ex0, ex1, ex2;
+  int nk2, freenode;
+  float a, b;
+};
+
+template <int XF, int YF, int ZF, int KIND, int AX>
+__global__ __launch_bounds__(256) void k_pml(PmlArgs g) {
+  const long li = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (li >= g.pcnt) return;
+  const int ii = (int)(li % g.ex0);
+  const long r = li / g.ex0;
+  const int jj = (int)(r % g.ex1);
+  const int kk = (int)(r / g.ex1);
+  const int i = g.lo0 + ii, j = g.lo1 + jj, k = g.lo2 + kk;
+  const long fs = g.fs, sy = g.sy, sz = g.sz;
+  const long c = i + sy * j + sz * (long)k;
+  const float* W = g.win;
+  const float rr = g.rrho[c];
+  const float tc13 = g.lam[c], tc66 = g.mu[c];
+  const float tc11 = tc13 + 2.0f * tc66, tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11;
+  const float tc44 = tc66, tc55 = tc66;
+  const float rzk = g.rz[k], rxs = g.rxsin[i], xc = g.xcot[i];
+  const int ax_idx = AX == 0 ? i : (AX == 1 ? j : k);
+  const float b1 = g.pb1[ax_idx], b1_1 = b1 - 1.0f, ad = g.pad[ax_idx], Dd = g.pdd[ax_idx];
+  float psi[6], dpsi[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) psi[v] = g.pin[v * g.pcnt + li];
+  float dh[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) dh[f] = 0.f;
+  const bool surf = g.freenode && k == g.nk2;
+
+  if constexpr (AX == 0) {
+    const float xi = g.xi_x[i];
+    float D;
+    D = (dop<XF>(W + TXX * fs + c, 1) * xi + 2.0f * W[TXZ * fs + c]) * rzk;
+    dh[VX] = (b1_1 * D - b1 * psi[3]) * rr;
+    dpsi[3] = Dd * D - ad * psi[3];
+    D = (dop<XF>(W + TXY * fs + c, 1) * xi + W[TYZ * fs + c]) * rzk;
+    dh[VY] = (b1_1 * D - b1 * psi[4]) * rr;
+    dpsi[4] = Dd * D - ad * psi[4];
+    D = (dop<XF>(W + TXZ * fs + c, 1) * xi - W[TXX * fs + c] + W[TZZ * fs + c]) * rzk;
+    dh[VZ] = (b1_1 * D - b1 * psi[5]) * rr;
+    dpsi[5] = Dd * D - ad * psi[5];
+    const float DxVx = (dop<XF>(W + VX * fs + c, 1) * xi + W[VZ * fs + c]) * rzk;
+    dh[TXX] = b1_1 * tc11 * DxVx - b1 * tc11 * psi[0];
+    dh[TYY] = b1_1 * tc12 * DxVx - b1 * tc12 * psi[0];
+    dh[TZZ] = b1_1 * tc13 * DxVx - b1 * tc13 * psi[0];
+    dpsi[0] = Dd * DxVx - ad * psi[0];
+    D = dop<XF>(W + VY * fs + c, 1) * xi * rzk;
+    dh[TXY] = b1_1 * tc66 * D - b1 * tc66 * psi[1];
+    dpsi[1] = Dd * D - ad * psi[1];
+    D = (dop<XF>(W + VZ * fs + c, 1) * xi - W[VX * fs + c]) * rzk;
+    dh[TXZ] = b1_1 * tc55 * D - b1 * tc55 * psi[2];
+    dpsi[2] = Dd * D - ad * psi[2];
+    if (surf) {  // mod_abs_npml.F90:1303-1333 (isotropic branch)
+      dh[TXX] += -b1_1 * tc13 * tc13 / tc33 * DxVx + b1 * tc13 * tc13 / tc33 * psi[0];
+      dh[TYY] += -b1_1 * tc23 * tc13 / tc33 * DxVx + b1 * tc23 * tc13 / tc33 * psi[0];
+    }
+  } else if constexpr (AX == 1) {
+    const float et = g.eta_y[j];
+    float D;
+    D = (dop<YF>(W + TXY * fs + c, sy) * et * rxs + (W[TXX * fs + c] - W[TYY * fs + c]) * xc + W[TXZ * fs + c]) * rzk;
+    dh[VX] = (b1_1 * D - b1 * psi[4]) * rr;
+    dpsi[4] = Dd * D - ad * psi[4];
+    D = (dop<YF>(W + TYY * fs + c, sy) * et * rxs + 2.0f * W[TXY * fs + c] * xc + 2.0f * W[TYZ * fs + c]) * rzk;
+    dh[VY] = (b1_1 * D - b1 * psi[3]) * rr;
+    dpsi[3] = Dd * D - ad * psi[3];
+    D = (dop<YF>(W + TYZ * fs + c, sy) * et * rxs + W[TXZ * fs + c] * xc + (W[TZZ * fs + c] - W[TYY * fs + c])) * rzk;
+    dh[VZ] = (b1_1 * D - b1 * psi[5]) * rr;
+    dpsi[5] = Dd * D - ad * psi[5];
+    D = (dop<YF>(W + VX * fs + c, sy) * et * rxs - W[VY * fs + c] * xc) * rzk;
+    dh[TXY] = b1_1 * tc66 * D - b1 * tc66 * psi[0];
+    dpsi[0] = Dd * D - ad * psi[0];
+    const float DyVy = (dop<YF>(W + VY * fs + c, sy) * et * rxs + W[VX * fs + c] * xc + W[VZ * fs + c]) * rzk;
+    dh[TXX] = b1_1 * tc12 * DyVy - b1 * tc12 * psi[1];
+    dh[TYY] = b1_1 * tc22 * DyVy - b1 * tc22 * psi[1];
+    dh[TZZ] = b1_1 * tc23 * DyVy - b1 * tc23 * psi[1];
+    dpsi[1] = Dd * DyVy - ad * psi[1];
+    D = (dop<YF>(W + VZ * fs + c, sy) * et * rxs - W[VY * fs + c]) * rzk;
+    dh[TYZ] = b1_1 * tc44 * D - b1 * tc44 * psi[2];
+    dpsi[2] = Dd * D - ad * psi[2];
+    if (surf) {
+      dh[TXX] += -b1_1 * tc13 * tc23 / tc33 * DyVy + b1 * tc13 * tc23 / tc33 * psi[1];
+      dh[TYY] += -b1_1 * tc23 * tc23 / tc33 * DyVy + b1 * tc23 * tc23 / tc33 * psi[1];
+    }
+  } else {
+    const float ze = g.zeta_z[k];
+    float D;
+    D = dop<ZF>(W + TXZ * fs + c, sz) * ze;
+    dh[VX] = (b1_1 * D - b1 * psi[4]) * rr;
+    dpsi[4] = Dd * D - ad * psi[4];
+    D = dop<ZF>(W + TYZ * fs + c, sz) * ze;
+    dh[VY] = (b1_1 * D - b1 * psi[5]) * rr;
+    dpsi[5] = Dd * D - ad * psi[5];
+    D = dop<ZF>(W + TZZ * fs + c, sz) * ze;
+    dh[VZ] = (b1_1 * D - b1 * psi[3]) * rr;
+    dpsi[3] = Dd * D - ad * psi[3];
+    D = dop<ZF>(W + VX * fs + c, sz) * ze;
+    dh[TXZ] = b1_1 * tc55 * D - b1 * tc55 * psi[0];
+    dpsi[0] = Dd * D - ad * psi[0];
+    D = dop<ZF>(W + VY * fs + c, sz) * ze;
+    dh[TYZ] = b1_1 * tc44 * D - b1 * tc44 * psi[1];
+    dpsi[1] = Dd * D - ad * psi[1];
+    D = dop<ZF>(W + VZ * fs + c, sz) * ze;
+    dh[TXX] = b1_1 * tc13 * D - b1 * tc13 * psi[2];
+    dh[TYY] = b1_1 * tc23 * D - b1 * tc23 * psi[2];
+    dh[TZZ] = b1_1 * tc33 * D - b1 * tc33 * psi[2];
+    dpsi[2] = Dd * D - ad * psi[2];
+  }
+  // fix-up of the stage outputs
+  constexpr unsigned MASK = AX == 0   ? ((1u << VX) | (1u << VY) | (1u << VZ) | (1u << TXX) | (1u << TYY) |
+                                       (1u << TZZ) | (1u << TXY) | (1u << TXZ))
+                            : AX == 1 ? ((1u << VX) | (1u << VY) | (1u << VZ) | (1u << TXX) | (1u << TYY) |
+                                       (1u << TZZ) | (1u << TXY) | (1u << TYZ))
+                                      : ((1u << VX) | (1u << VY) | (1u << VZ) | (1u << TXX) | (1u << TYY) |
+                                       (1u << TZZ) | (1u << TXZ) | (1u << TYZ));
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    if (!(MASK & (1u << f))) continue;
+    if constexpr (KIND == 2) {
+      g.wout[f * fs + c] += g.b * dh[f];
+    } else {
+      g.wout[f * fs + c] += g.a * dh[f];
+      g.t[f * fs + c] += g.b * dh[f];
+    }
+  }
+  // auxiliary RK
+#pragma unroll
+  for (int v = 0; v < 6; ++v) {
+    const long q = v * g.pcnt + li;
+    if constexpr (KIND == 0) {
+      g.pout[q] = psi[v] + g.a * dpsi[v];
+      g.pt[q] = psi[v] + g.b * dpsi[v];
+    } else if constexpr (KIND == 1) {
+      g.pout[q] = g.pm[q] + g.a * dpsi[v];
+      g.pt[q] = g.pt[q] + g.b * dpsi[v];
+    } else {
+      g.pout[q] = g.pt[q] + g.b * dpsi[v];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Point-source terms as an RK fix-up, one thread per affected cell:
+//   type 0  smoothed body force  hV += F*stf*w,  w = d/(V rho)   (src_force)
+//   type 1  moment source        hT -= M*rate*w, w = d/V          (src_stress)
+//   type 2  surface traction rate of a surface force at k = nk2 (the VSrc
+//           terms of the traction condition, mod_macdrp.F90:942-950)
+// ---------------------------------------------------------------------------
+struct SrcContrib {
+  int type, n, m;  // m: window for type 2 (last writer); unused otherwise
+  float w;         // weight; for type 2: 1/S
+};
+struct SrcArgs {
+  float* wout;
+  float* t;
+  const float* __restrict__ lam;
+  const float* __restrict__ mu;
+  const long* __restrict__ cell;
+  const int* __restrict__ ptr;
+  const SrcContrib* __restrict__ con;
+  // force tables
+  const float* __restrict__ ft0;
+  const float* __restrict__ ftime;
+  const float* __restrict__ ffreq;
+  const float* __restrict__ F;  // [3][ntw*n]
+  long fblk;
+  int fntw, fstf;
+  // moment tables
+  const float* __restrict__ mt0;
+  const float* __restrict__ mtime;
+  const float* __restrict__ mfreq;
+  const float* __restrict__ M;  // [6][ntw*n]
+  long mblk;
+  int mntw, mstf;
+  long fs;
+  int ncell, ntime, kind;
+  float ts, stept, a, b;
+};
+__global__ void k_src(SrcArgs g) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= g.ncell) return;
+  const long c = g.cell[u];
+  float dh[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) dh[f] = 0.f;
+  for (int q = g.ptr[u]; q < g.ptr[u + 1]; ++q) {
+    const SrcContrib s = g.con[q];
+    if (s.type == 0) {
+      for (int m = 0; m < g.fntw; ++m) {
+        const float disp = stf_eval(0, g.fstf, g.ts - g.ft0[s.n], g.ftime[m], g.ffreq[m], g.ntime, g.stept);
+        const long qq = m + (long)g.fntw * s.n;
+        dh[VX] += g.F[qq] * disp * s.w;
+        dh[VY] += g.F[g.fblk + qq] * disp * s.w;
+        dh[VZ] += g.F[2 * g.fblk + qq] * disp * s.w;
+      }
+    } else if (s.type == 1) {
+      for (int m = 0; m < g.mntw; ++m) {
+        const float rate = stf_eval(1, g.mstf, g.ts - g.mt0[s.n], g.mtime[m], g.mfreq[m], g.ntime, g.stept);
+        const float v = s.w * rate;
+        const long qq = m + (long)g.mntw * s.n;
+        dh[TXX] -= g.M[qq] * v;
+        dh[TYY] -= g.M[g.mblk + qq] * v;
+        dh[TZZ] -= g.M[2 * g.mblk + qq] * v;
+        dh[TXY] -= g.M[3 * g.mblk + qq] * v;
+        dh[TXZ] -= g.M[4 * g.mblk + qq] * v;
+        dh[TYZ] -= g.M[5 * g.mblk + qq] * v;
+      }
+    } else {
+      const float rate = stf_eval(1, g.fstf, g.ts - g.ft0[s.n], g.ftime[s.m], g.ffreq[s.m], g.ntime, g.stept);
+      const long qq = s.m + (long)g.fntw * s.n;
+      const float vx = g.F[qq] * s.w * rate, vy = g.F[g.fblk + qq] * s.w * rate,
+                  vz = g.F[2 * g.fblk + qq] * s.w * rate;
+      const float lam = g.lam[c], miu = g.mu[c], lam2mu = lam + 2.0f * miu;
+      const float dE33 = vz / lam2mu, dE13 = vx / miu / 2.0f, dE23 = vy / miu / 2.0f;
+      dh[TXX] += lam * dE33;
+      dh[TYY] += lam * dE33;
+      dh[TZZ] += lam2mu * dE33;
+      dh[TXZ] += 2.0f * miu * dE13;
+      dh[TYZ] += 2.0f * miu * dE23;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    if (g.kind == 2) {
+      g.wout[f * g.fs + c] += g.b * dh[f];
+    } else {
+      g.wout[f * g.fs + c] += g.a * dh[f];
+      g.t[f * g.fs + c] += g.b * dh[f];
+    }
+  }
+}
+
+// receivers: rec[record][p][9]
+__global__ void k_receivers(const float* __restrict__ W, long fs, const long* __restrict__ pt, int npt,
+                            float* __restrict__ rec) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npt * 9) return;
+  const int p = q / 9, f = q % 9;
+  rec[q] = W[f * fs + pt[p]];
+}
+
+// snapshot: out[nf][npts] for the fields in the mask list
+struct SnapArgs {
+  const float* __restrict__ W;
+  float* __restrict__ out;
+  long fs, sy, sz, npts;
+  int s0, s1, s2, c0, c1, t0, t1, t2;
+  int nf;
+  int flist[9];
+};
+__global__ void k_snapshot(SnapArgs g) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.npts) return;
+  const int a = (int)(p % g.c0);
+  const long r = p / g.c0;
+  const int b = (int)(r % g.c1);
+  const int d = (int)(r / g.c1);
+  const long c = (g.s0 + a * g.t0) + g.sy * (long)(g.s1 + b * g.t1) + g.sz * (long)(g.s2 + d * g.t2);
+  for (int q = 0; q < g.nf; ++q) g.out[q * g.npts + p] = g.W[g.flist[q] * g.fs + c];
+}
+
+// FP64 metric construction (seis3d_metric.F90:82-112): sin/cot of colatitude
+// and the inverse of x_xi = (B.x + F.x)/2; ghost values copied from the edge.
+__global__ void k_metric(const double* __restrict__ g, int n, int lo, int hi, float* __restrict__ inv,
+                         const double* __restrict__ gx_for_trig, float* __restrict__ xsin, float* __restrict__ xcot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (xsin) {
+    const double th = gx_for_trig[i];
+    xsin[i] = (float)sin(th);
+    xcot[i] = (float)(1.0 / tan(th));
+  }
+  const double cF[5] = {-0.30874, -0.6326, 1.2330, -0.3334, 0.04168};
+  const double cB[5] = {-0.04168, 0.3334, -1.2330, 0.6326, 0.30874};
+  const int ic = i < lo ? lo : (i > hi ? hi : i);
+  double b = 0.0, f = 0.0;
+  for (int q = 0; q < 5; ++q) {
+    b += g[ic - 3 + q] * cB[q];
+    f += g[ic - 1 + q] * cF[q];
+  }
+  inv[i] = (float)(1.0 / ((b + f) * 0.5));
+}
+
+template <typename T>
+__global__ void k_fill(T* p, long n, T v) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) p[q] = v;
+}
+
+// 8-block operator cycle, seis3d_wave_8op_p3.F90:82-591 (1 = forward)
+const int BLOCK_PAT[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+struct PmlFace {
+  bool on = false;
+  int n = 0, lo[3]{}, hi[3]{}, ex[3]{};
+  long cnt = 0;
+  float* d = nullptr;  // 4 sets x 6 vars x cnt
+};
+struct Snapshot {
+  int subs[3], subc[3], subt[3], tinv, nf, flist[9];
+  long npts;
+  int max_records, nrec = 0;
+  float* d = nullptr;
+};
+
+struct fw_solver_s {
+  fw_config cfg{};
+  int nx, ny, nz, px;
+  long sy, sz, fs, ps;  // pitched strides; ps = plane stride of 2-D arrays
+  long off;             // leading pad so that column ni1 is 64-byte aligned
+  int ni1, ni2, nj1, nj2, nk1, nk2;
+  int freenode;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool ready = false;
+
+  float* d_sets = nullptr;  // 4 * 9 * fs (+ off)
+  float* set[4]{};          // A B C T
+  float* d_media = nullptr; // lam mu rrho
+  float *lam = nullptr, *mu = nullptr, *rrho = nullptr;
+  float* d_1d = nullptr;
+  float *xi_x, *eta_y, *zeta_z, *rz, *rxsin, *xcot;
+  float* d_msurf = nullptr;
+  // host copies
+  std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
+  std::vector<float> hlam, hmu, hrho;
+  bool have_grid = false, have_media = false;
+  // PML
+  std::vector<float> Ax, Bx, Dx, Ay, By, Dy, Az, Bz, Dz;
+  float* d_pml = nullptr;  // b1, ad, dd for x, y, z
+  PmlFace face[6];
+  // sources
+  int nforce = 0, fntw = 0, fstf = 0;
+  std::vector<int> findx;
+  std::vector<float> fshift, ft0, ftime, ffreq, F;
+  int nmom = 0, mntw = 0, mstf = 0;
+  std::vector<int> mindx;
+  std::vector<float> mshift, mt0, mtime, mfreq, M;
+  void* d_src = nullptr;  // packed source tables
+  std::vector<void*> d_src_allocs;
+  float *d_ft0 = nullptr, *d_ftime = nullptr, *d_ffreq = nullptr, *d_F = nullptr;
+  float *d_mt0 = nullptr, *d_mtime = nullptr, *d_mfreq = nullptr, *d_M = nullptr;
+  int* d_smap = nullptr;
+  SurfEntry* d_sent = nullptr;
+  long* d_scell = nullptr;
+  int* d_sptr = nullptr;
+  SrcContrib* d_scon = nullptr;
+  int n_srccell = 0, n_surf = 0;
+  // receivers
+  int npt = 0, pt_tinv = 1, max_rec = 0, nrec = 0;
+  std::vector<int> pt_indx;
+  long* d_pt = nullptr;
+  float* d_rec = nullptr;
+  std::vector<Snapshot> snaps;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double stage_ms = 0, total_ms = 0;
+  int nstage = 0;
+};
+
+static int fail(fw_handle h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  else g_create_err = msg;
+  return code;
+}
+
+extern "C" {
+
+const char* fw_last_error(fw_handle h) { return h ? h->err.c_str() : g_create_err.c_str(); }
+
+int fw_reference_steps(int nt) { return nt < 0 ? 0 : 8 * ((nt + 1 + 7) / 8); }
+
+int fw_create(const fw_config* cfg, fw_handle* out) {
+  fw_handle h = nullptr;
+  if (!cfg || !out) return fail(nullptr, FW_ERR_ARG, "fw_create: null argument");
+  if (cfg->ni < 8 || cfg->nj < 8 || cfg->nk < 8) return fail(nullptr, FW_ERR_ARG, "fw_create: ni, nj, nk must be >= 8");
+  if (!(cfg->stept > 0.f)) return fail(nullptr, FW_ERR_ARG, "fw_create: stept must be > 0");
+  for (int n = 0; n < 6; ++n) {
+    const int N = cfg->abs_number[n];
+    const int ext = n < 2 ? cfg->ni : (n < 4 ? cfg->nj : cfg->nk);
+    if (N < 0 || N + 1 > ext) return fail(nullptr, FW_ERR_ARG, "fw_create: abs_number out of range");
+    if (N > 0 && cfg->cfs_bmax[n] < 1.0f)
+      return fail(nullptr, FW_ERR_ARG, "fw_create: CFS_bmax should be large or equal to 1");
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, FW_ERR_HIP, "fw_create: no HIP device");
+  h = new fw_solver_s();
+  h->cfg = *cfg;
+  h->nx = cfg->ni + 6; h->ny = cfg->nj + 6; h->nz = cfg->nk + 6;
+  h->px = (h->nx + 15) / 16 * 16;
+  h->sy = h->px; h->sz = (long)h->px * h->ny; h->ps = h->sz;
+  h->off = 13;  // (off + ni1) % 16 == 0 with ni1 = 3
+  h->fs = h->sz * h->nz + 16;
+  h->ni1 = 3; h->ni2 = 3 + cfg->ni - 1; h->nj1 = 3; h->nj2 = 3 + cfg->nj - 1; h->nk1 = 3; h->nk2 = 3 + cfg->nk - 1;
+  h->freenode = cfg->freenode ? 1 : 0;
+  if (cfg->abs_number[5] > 0) h->freenode = 0;  // mod_abs_npml.F90:182-184
+  hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete h; return fail(nullptr, FW_ERR_HIP, "hipStreamCreate failed"); }
+  const size_t setbytes = sizeof(float) * (size_t)(9 * h->fs);
+  e = hipMalloc(&h->d_sets, 4 * setbytes + 64 * sizeof(float));
+  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc wavefield sets failed"); }
+  for (int s = 0; s < 4; ++s) h->set[s] = h->d_sets + h->off + (size_t)s * 9 * h->fs;
+  e = hipMalloc(&h->d_media, sizeof(float) * (size_t)(3 * h->fs + 64));
+  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc media failed"); }
+  h->lam = h->d_media + h->off; h->mu = h->lam + h->fs; h->rrho = h->mu + h->fs;
+  const long n1 = h->nx + h->ny + h->nz;
+  e = hipMalloc(&h->d_1d, sizeof(float) * 2 * n1 + sizeof(float) * 4 * (h->nx + h->nz));
+  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc 1-d failed"); }
+  h->xi_x = h->d_1d; h->eta_y = h->xi_x + h->nx; h->zeta_z = h->eta_y + h->ny;
+  h->rz = h->zeta_z + h->nz; h->rxsin = h->rz + h->nz; h->xcot = h->rxsin + h->nx;
+  e = hipMalloc(&h->d_msurf, sizeof(float) * 3 * h->ps);
+  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc surface failed"); }
+  *out = h;
+  return FW_OK;
+}
+
+int fw_destroy(fw_handle h) {
+  if (!h) return FW_OK;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  hipFree(h->d_sets); hipFree(h->d_media); hipFree(h->d_1d); hipFree(h->d_msurf); hipFree(h->d_pml);
+  for (auto& f : h->face) hipFree(f.d);
+  for (void* p : h->d_src_allocs) hipFree(p);
+  hipFree(h->d_pt); hipFree(h->d_rec);
+  for (auto& s : h->snaps) hipFree(s.d);
+  for (auto& e : h->ev) hipEventDestroy(e);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return FW_OK;
+}
+
+int fw_dims(fw_handle h, int* nx, int* ny, int* nz, int* freenode) {
+  if (!h) return FW_ERR_ARG;
+  if (nx) *nx = h->nx;
+  if (ny) *ny = h->ny;
+  if (nz) *nz = h->nz;
+  if (freenode) *freenode = h->freenode;
+  return FW_OK;
+}
+
+static int upload_1d(fw_handle h) {
+  std::vector<float> rz(h->nz), rxs(h->nx);
+  for (int k = 0; k < h->nz; ++k) rz[k] = (float)(1.0 / (double)h->hz[k]);
+  for (int i = 0; i < h->nx; ++i) rxs[i] = (float)(1.0 / (double)h->hxsin[i]);
+  HIPCHK(hipMemcpyAsync(h->xi_x, h->hxi.data(), sizeof(float) * h->nx, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->eta_y, h->het.data(), sizeof(float) * h->ny, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->zeta_z, h->hze.data(), sizeof(float) * h->nz, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->rz, rz.data(), sizeof(float) * h->nz, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->rxsin, rxs.data(), sizeof(float) * h->nx, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(h->xcot, h->hxcot.data(), sizeof(float) * h->nx, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+int fw_set_grid(fw_handle h, const float* x, const float* y, const float* z, const float* xsin, const float* xcot,
+                const float* xi_x, const float* eta_y, const float* zeta_z) {
+  if (!h || !x || !y || !z || !xsin || !xcot || !xi_x || !eta_y || !zeta_z) return fail(h, FW_ERR_ARG, "fw_set_grid: null argument");
+  h->hx.assign(x, x + h->nx); h->hy.assign(y, y + h->ny); h->hz.assign(z, z + h->nz);
+  h->hxsin.assign(xsin, xsin + h->nx); h->hxcot.assign(xcot, xcot + h->nx);
+  h->hxi.assign(xi_x, xi_x + h->nx); h->het.assign(eta_y, eta_y + h->ny); h->hze.assign(zeta_z, zeta_z + h->nz);
+  for (int k = 0; k < h->nz; ++k)
+    if (!(h->hz[k] != 0.f)) return fail(h, FW_ERR_ARG, "fw_set_grid: z must be nonzero");
+  for (int i = 0; i < h->nx; ++i)
+    if (!(h->hxsin[i] != 0.f)) return fail(h, FW_ERR_ARG, "fw_set_grid: xsin must be nonzero");
+  h->have_grid = true;
+  h->ready = false;
+  return upload_1d(h);
+}
+
+int fw_build_metric(fw_handle h, const double* x, const double* y, const double* z, float* oxs, float* oxc,
+                    float* oxi, float* oet, float* oze) {
+  if (!h || !x || !y || !z) return fail(h, FW_ERR_ARG, "fw_build_metric: null argument");
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  double* d = nullptr;
+  float* f = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(double) * (nx + ny + nz)));
+  if (hipMalloc(&f, sizeof(float) * (3 * nx + ny + nz)) != hipSuccess) {
+    hipFree(d);
+    return fail(h, FW_ERR_NOMEM, "fw_build_metric: hipMalloc failed");
+  }
+  hipMemcpyAsync(d, x, sizeof(double) * nx, hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(d + nx, y, sizeof(double) * ny, hipMemcpyHostToDevice, h->stream);
+  hipMemcpyAsync(d + nx + ny, z, sizeof(double) * nz, hipMemcpyHostToDevice, h->stream);
+  float *fxs = f, *fxc = f + nx, *fxi = f + 2 * nx, *fet = f + 3 * nx, *fze = fet + ny;
+  hipLaunchKernelGGL(k_metric, dim3((nx + 63) / 64), dim3(64), 0, h->stream, d, nx, 3, nx - 4, fxi, d, fxs, fxc);
+  hipLaunchKernelGGL(k_metric, dim3((ny + 63) / 64), dim3(64), 0, h->stream, d + nx, ny, 3, ny - 4, fet,
+                     (const double*)nullptr, (float*)nullptr, (float*)nullptr);
+  hipLaunchKernelGGL(k_metric, dim3((nz + 63) / 64), dim3(64), 0, h->stream, d + nx + ny, nz, 3, nz - 4, fze,
+                     (const double*)nullptr, (float*)nullptr, (float*)nullptr);
+  std::vector<float> hf(3 * nx + ny + nz);
+  hipError_t e = hipMemcpyAsync(hf.data(), f, sizeof(float) * hf.size(), hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d);
+  hipFree(f);
+  if (e != hipSuccess) return fail(h, FW_ERR_HIP, std::string("fw_build_metric: ") + hipGetErrorString(e));
+  h->hx.resize(nx); h->hy.resize(ny); h->hz.resize(nz);
+  for (int i = 0; i < nx; ++i) h->hx[i] = (float)x[i];
+  for (int j = 0; j < ny; ++j) h->hy[j] = (float)y[j];
+  for (int k = 0; k < nz; ++k) h->hz[k] = (float)z[k];
+  h->hxsin.assign(hf.begin(), hf.begin() + nx);
+  h->hxcot.assign(hf.begin() + nx, hf.begin() + 2 * nx);
+  h->hxi.assign(hf.begin() + 2 * nx, hf.begin() + 3 * nx);
+  h->het.assign(hf.begin() + 3 * nx, hf.begin() + 3 * nx + ny);
+  h->hze.assign(hf.begin() + 3 * nx + ny, hf.end());
+  if (oxs) std::copy(h->hxsin.begin(), h->hxsin.end(), oxs);
+  if (oxc) std::copy(h->hxcot.begin(), h->hxcot.end(), oxc);
+  if (oxi) std::copy(h->hxi.begin(), h->hxi.end(), oxi);
+  if (oet) std::copy(h->het.begin(), h->het.end(), oet);
+  if (oze) std::copy(h->hze.begin(), h->hze.end(), oze);
+  h->have_grid = true;
+  h->ready = false;
+  return upload_1d(h);
+}
+
+static int upload_volume(fw_handle h, float* dst, const float* src) {
+  HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * h->px, src, sizeof(float) * h->nx, sizeof(float) * h->nx,
+                          (size_t)h->ny * h->nz, hipMemcpyHostToDevice, h->stream));
+  return FW_OK;
+}
+static int download_volume(fw_handle h, float* dst, const float* src) {
+  HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * h->nx, src, sizeof(float) * h->px, sizeof(float) * h->nx,
+                          (size_t)h->ny * h->nz, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+int fw_set_media(fw_handle h, const float* lam, const float* mu, const float* rho) {
+  if (!h || !lam || !mu || !rho) return fail(h, FW_ERR_ARG, "fw_set_media: null argument");
+  const size_t n = (size_t)h->nx * h->ny * h->nz;
+  h->hlam.assign(lam, lam + n); h->hmu.assign(mu, mu + n); h->hrho.assign(rho, rho + n);
+  std::vector<float> rr(n);
+  for (size_t q = 0; q < n; ++q) {
+    if (!(rho[q] > 0.f)) return fail(h, FW_ERR_ARG, "fw_set_media: rho must be > 0 everywhere");
+    rr[q] = 1.0f / rho[q];
+  }
+  int rc;
+  if ((rc = upload_volume(h, h->lam, lam))) return rc;
+  if ((rc = upload_volume(h, h->mu, mu))) return rc;
+  if ((rc = upload_volume(h, h->rrho, rr.data()))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->have_media = true;
+  h->ready = false;
+  return FW_OK;
+}
+
+int fw_set_force(fw_handle h, int n, int ntw, int stf_id, const int* indx, const float* shift, const float* t0,
+                 const float* stf_time, const float* stf_freq, const float* Fx, const float* Fy, const float* Fz) {
+  if (!h || n < 0 || (n > 0 && (ntw <= 0 || !indx || !shift || !t0 || !stf_time || !stf_freq || !Fx || !Fy || !Fz)))
+    return fail(h, FW_ERR_ARG, "fw_set_force: bad argument");
+  h->nforce = n; h->fntw = ntw; h->fstf = stf_id;
+  h->findx.assign(indx, indx + 3 * n);
+  h->fshift.assign(shift, shift + 3 * n);
+  h->ft0.assign(t0, t0 + n);
+  h->ftime.assign(stf_time, stf_time + ntw);
+  h->ffreq.assign(stf_freq, stf_freq + ntw);
+  h->F.resize(3 * (size_t)ntw * n);
+  std::copy(Fx, Fx + ntw * n, h->F.begin());
+  std::copy(Fy, Fy + ntw * n, h->F.begin() + ntw * n);
+  std::copy(Fz, Fz + ntw * n, h->F.begin() + 2 * ntw * n);
+  h->ready = false;
+  return FW_OK;
+}
+
+int fw_set_moment(fw_handle h, int n, int ntw, int stf_id, const int* indx, const float* shift, const float* t0,
+                  const float* stf_time, const float* stf_freq, const float* M6) {
+  if (!h || n < 0 || (n > 0 && (ntw <= 0 || !indx || !shift || !t0 || !stf_time || !stf_freq || !M6)))
+    return fail(h, FW_ERR_ARG, "fw_set_moment: bad argument");
+  h->nmom = n; h->mntw = ntw; h->mstf = stf_id;
+  h->mindx.assign(indx, indx + 3 * n);
+  h->mshift.assign(shift, shift + 3 * n);
+  h->mt0.assign(t0, t0 + n);
+  h->mtime.assign(stf_time, stf_time + ntw);
+  h->mfreq.assign(stf_freq, stf_freq + ntw);
+  h->M.assign(M6, M6 + 6 * (size_t)ntw * n);
+  h->ready = false;
+  return FW_OK;
+}
+
+int fw_set_receivers(fw_handle h, int n, int tinv, const int* indx, int max_records) {
+  if (!h || n < 0 || (n > 0 && (!indx || tinv <= 0 || max_records < 0))) return fail(h, FW_ERR_ARG, "fw_set_receivers: bad argument");
+  for (int p = 0; p < n; ++p) {
+    const int i = indx[3 * p] - 1, j = indx[3 * p + 1] - 1, k = indx[3 * p + 2] - 1;
+    if (i < 0 || i >= h->nx || j < 0 || j >= h->ny || k < 0 || k >= h->nz)
+      return fail(h, FW_ERR_ARG, "fw_set_receivers: receiver index outside the ghosted grid");
+  }
+  hipFree(h->d_pt); hipFree(h->d_rec);
+  h->d_pt = nullptr; h->d_rec = nullptr;
+  h->npt = n; h->pt_tinv = tinv > 0 ? tinv : 1; h->max_rec = max_records; h->nrec = 0;
+  h->pt_indx.assign(indx, indx + 3 * n);
+  if (n > 0) {
+    std::vector<long> lin(n);
+    for (int p = 0; p < n; ++p)
+      lin[p] = (indx[3 * p] - 1) + h->sy * (long)(indx[3 * p + 1] - 1) + h->sz * (long)(indx[3 * p + 2] - 1);
+    HIPCHK(hipMalloc(&h->d_pt, sizeof(long) * n));
+    HIPCHK(hipMemcpy(h->d_pt, lin.data(), sizeof(long) * n, hipMemcpyHostToDevice));
+    if (max_records > 0) HIPCHK(hipMalloc(&h->d_rec, sizeof(float) * 9 * (size_t)n * max_records));
+  }
+  return FW_OK;
+}
+
+int fw_add_snapshot(fw_handle h, const int* subs, const int* subc, const int* subt, int tinv, unsigned mask,
+                    int max_records, int* snap_id) {
+  if (!h || !subs || !subc || !subt || tinv <= 0 || max_records <= 0 || !(mask & 0x1ff) || !snap_id)
+    return fail(h, FW_ERR_ARG, "fw_add_snapshot: bad argument");
+  Snapshot s{};
+  const int ext[3] = {h->nx, h->ny, h->nz};
+  for (int d = 0; d < 3; ++d) {
+    s.subs[d] = subs[d] - 1; s.subc[d] = subc[d]; s.subt[d] = subt[d];
+    if (subc[d] <= 0 || subt[d] <= 0 || s.subs[d] < 0 || s.subs[d] + (subc[d] - 1) * subt[d] >= ext[d])
+      return fail(h, FW_ERR_ARG, "fw_add_snapshot: sub-volume outside the ghosted grid");
+  }
+  s.tinv = tinv;
+  s.nf = 0;
+  for (int f = 0; f < 9; ++f)
+    if (mask & (1u << f)) s.flist[s.nf++] = f;
+  s.npts = (long)subc[0] * subc[1] * subc[2];
+  s.max_records = max_records;
+  HIPCHK(hipMalloc(&s.d, sizeof(float) * (size_t)s.nf * s.npts * max_records));
+  h->snaps.push_back(s);
+  *snap_id = (int)h->snaps.size() - 1;
+  return FW_OK;
+}
+
+// PML profiles, abs_init (mod_abs_npml.F90:135-300) with the helpers of
+// :9070-9124, evaluated in float like the reference.
+static float pml_R(int n) { return (float)std::pow(10.0, -(std::log10((double)n) - 1.0) / std::log10(2.0) - 3.0); }
+static float pml_dmax(float L, float Vp, float Rpp) { return -Vp / 2.0f / L * (float)std::log((double)Rpp) * (2.0f + 1.0f); }
+static float pml_d(float x, float L, float dmax) { return x < 0 ? 0.f : dmax * ((x / L) * (x / L)); }
+static float pml_a(float x, float L, float amax) { return x < 0 ? 0.f : amax * (1.0f - (x / L)); }
+static float pml_b(float x, float L, float bmax) { return x < 0 ? 1.f : 1.0f + (bmax - 1.0f) * ((x / L) * (x / L)); }
+
+static void build_pml(fw_handle h) {
+  const fw_config& c = h->cfg;
+  h->Ax.assign(h->nx, 0.f); h->Bx.assign(h->nx, 1.f); h->Dx.assign(h->nx, 0.f);
+  h->Ay.assign(h->ny, 0.f); h->By.assign(h->ny, 1.f); h->Dy.assign(h->ny, 0.f);
+  h->Az.assign(h->nz, 0.f); h->Bz.assign(h->nz, 1.f); h->Dz.assign(h->nz, 0.f);
+  const std::vector<float>&x = h->hx, &y = h->hy, &z = h->hz;
+  const float ztop = z[h->nk2];
+  int N;
+  if ((N = c.abs_number[0]) > 0) {
+    const float x0 = x[N + h->ni1], L0 = ztop * (x0 - x[h->ni1]);
+    const float dm = pml_dmax(L0, c.abs_velocity[0], pml_R(N));
+    for (int i = h->ni1; i <= N + h->ni1; ++i) {
+      const float L = ztop * (x0 - x[i]);
+      h->Dx[i] = pml_d(L, L0, dm); h->Ax[i] = pml_a(L, L0, c.cfs_amax[0]); h->Bx[i] = pml_b(L, L0, c.cfs_bmax[0]);
+    }
+  }
+  if ((N = c.abs_number[1]) > 0) {
+    const float x0 = x[h->ni2 - N], L0 = ztop * (x[h->ni2] - x0);
+    const float dm = pml_dmax(L0, c.abs_velocity[1], pml_R(N));
+    for (int i = h->ni2 - N; i <= h->ni2; ++i) {
+      const float L = ztop * (x[i] - x0);
+      h->Dx[i] = pml_d(L, L0, dm); h->Ax[i] = pml_a(L, L0, c.cfs_amax[1]); h->Bx[i] = pml_b(L, L0, c.cfs_bmax[1]);
+    }
+  }
+  if ((N = c.abs_number[2]) > 0) {
+    const float y0 = y[N + h->nj1], L0 = ztop * (y0 - y[h->nj1]);
+    const float dm = pml_dmax(L0, c.abs_velocity[2], pml_R(N));
+    for (int j = h->nj1; j <= N + h->nj1; ++j) {
+      const float L = ztop * (y0 - y[j]);
+      h->Dy[j] = pml_d(L, L0, dm); h->Ay[j] = pml_a(L, L0, c.cfs_amax[2]); h->By[j] = pml_b(L, L0, c.cfs_bmax[2]);
+    }
+  }
+  if ((N = c.abs_number[3]) > 0) {
+    const float y0 = y[h->nj2 - N], L0 = ztop * (y[h->nj2] - y0);
+    const float dm = pml_dmax(L0, c.abs_velocity[3], pml_R(N));
+    for (int j = h->nj2 - N; j <= h->nj2; ++j) {
+      const float L = ztop * (y[j] - y0);
+      h->Dy[j] = pml_d(L, L0, dm); h->Ay[j] = pml_a(L, L0, c.cfs_amax[3]); h->By[j] = pml_b(L, L0, c.cfs_bmax[3]);
+    }
+  }
+  if ((N = c.abs_number[4]) > 0) {
+    const float z0 = z[N + h->nk1], L0 = z0 - z[h->nk1];
+    const float dm = pml_dmax(L0, c.abs_velocity[4], pml_R(N));
+    for (int k = h->nk1; k <= h->nk1 + N; ++k) {
+      const float L = z0 - z[k];
+      h->Dz[k] = pml_d(L, L0, dm); h->Az[k] = pml_a(L, L0, c.cfs_amax[4]); h->Bz[k] = pml_b(L, L0, c.cfs_bmax[4]);
+    }
+  }
+  if ((N = c.abs_number[5]) > 0) {
+    const float z0 = z[h->nk2 - N], L0 = z[h->nk2] - z0;
+    const float dm = pml_dmax(L0, c.abs_velocity[5], pml_R(N));
+    for (int k = h->nk2 - N; k <= h->nk2; ++k) {
+      const float L = z[k] - z0;
+      h->Dz[k] = pml_d(L, L0, dm); h->Az[k] = pml_a(L, L0, c.cfs_amax[5]); h->Bz[k] = pml_b(L, L0, c.cfs_bmax[5]);
+    }
+  }
+  for (int i = 0; i < h->nx; ++i) h->Dx[i] = h->Dx[i] / h->Bx[i];
+  for (int j = 0; j < h->ny; ++j) h->Dy[j] = h->Dy[j] / h->By[j];
+  for (int k = 0; k < h->nz; ++k) h->Dz[k] = h->Dz[k] / h->Bz[k];
+}
+
+static int build_sources(fw_handle h) {
+  for (void* p : h->d_src_allocs) hipFree(p);
+  h->d_src_allocs.clear();
+  h->n_srccell = 0;
+  h->n_surf = 0;
+  auto dalloc = [&](void** p, size_t bytes, const void* src) -> int {
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(p, bytes) != hipSuccess) return -1;
+    h->d_src_allocs.push_back(*p);
+    if (src && hipMemcpy(*p, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    return 0;
+  };
+  const std::vector<float>&x = h->hx, &y = h->hy, &z = h->hz;
+  auto volume = [&](int i, int j, int k) {
+    return (x[i + 1] - x[i - 1]) * (y[j + 1] - y[j - 1]) * (z[k + 1] - z[k - 1]) / 8.0f * (z[k] * z[k]) * h->hxsin[i];
+  };
+  auto norm_delt = [&](float* d, float x0, float y0, float z0, int sk) {
+    const float r0 = 1.5f;
+    for (int k = -3; k <= 3; ++k)
+      for (int j = -3; j <= 3; ++j)
+        for (int i = -3; i <= 3; ++i)
+          d[(i + 3) + 7 * (j + 3) + 49 * (k + 3)] =
+              fun_gauss((float)i - x0, r0, 0.f) * fun_gauss((float)j - y0, r0, 0.f) * fun_gauss((float)k - z0, r0, 0.f);
+    float s = 0.f;
+    for (int q = 0; q < 343; ++q) s += d[q];
+    for (int q = 0; q < 343; ++q) d[q] = d[q] / s;
+    if (h->freenode && sk + 3 > h->nk2) {  // mod_src.F90:230-232
+      float s2 = 0.f;
+      for (int k = -3; k <= h->nk2 - sk; ++k)
+        for (int q = 0; q < 49; ++q) s2 += d[q + 49 * (k + 3)];
+      for (int q = 0; q < 343; ++q) d[q] = d[q] / s2;
+    }
+  };
+  std::map<long, std::vector<SrcContrib>> cells;
+  std::vector<int> smap((size_t)h->ps, -1);
+  std::vector<SurfEntry> sent;
+  std::map<long, int> surf_of;  // plane index -> entry
+  float d[343];
+  for (int n = 0; n < h->nforce; ++n) {
+    const int si = h->findx[3 * n] - 1, sj = h->findx[3 * n + 1] - 1, sk = h->findx[3 * n + 2] - 1;
+    // surface traction entries (src_surface), last (n, m) wins
+    if (h->freenode && si >= h->ni1 && si <= h->ni2 && sj >= h->nj1 && sj <= h->nj2 && sk == h->nk2) {
+      const float S = (x[si + 1] - x[si - 1]) * (y[sj + 1] - y[sj - 1]) / 4.0f * (z[sk] * z[sk]) * h->hxsin[si];
+      const long c2 = si + h->sy * (long)sj;
+      auto it = surf_of.find(c2);
+      SurfEntry e{n, h->fntw - 1, 0.f, S};
+      if (it == surf_of.end()) {
+        surf_of[c2] = (int)sent.size();
+        sent.push_back(e);
+      } else {
+        sent[it->second] = e;
+      }
+    }
+    norm_delt(d, h->fshift[3 * n], h->fshift[3 * n + 1], h->fshift[3 * n + 2], sk);
+    for (int Lk = -3; Lk <= 3; ++Lk)
+      for (int Lj = -3; Lj <= 3; ++Lj)
+        for (int Li = -3; Li <= 3; ++Li) {
+          const int i = Li + si, j = Lj + sj, k = Lk + sk;
+          if (i < h->ni1 || i > h->ni2 || j < h->nj1 || j > h->nj2 || k < h->nk1 || k > h->nk2) continue;
+          if (h->freenode && k == h->nk2) continue;  // SrcSurface: surface plane carried by traction
+          const long c = i + h->sy * (long)j + h->sz * (long)k;
+          const float V = volume(i, j, k);
+          const float w = 1.0f / (V * h->hrho[i + (size_t)h->nx * (j + (size_t)h->ny * k)]) * d[(Li + 3) + 7 * (Lj + 3) + 49 * (Lk + 3)];
+          cells[c].push_back(SrcContrib{0, n, 0, w});
+        }
+  }
+  for (int n = 0; n < h->nmom; ++n) {
+    const int si = h->mindx[3 * n] - 1, sj = h->mindx[3 * n + 1] - 1, sk = h->mindx[3 * n + 2] - 1;
+    norm_delt(d, h->mshift[3 * n], h->mshift[3 * n + 1], h->mshift[3 * n + 2], sk);
+    for (int Lk = -3; Lk <= 3; ++Lk)
+      for (int Lj = -3; Lj <= 3; ++Lj)
+        for (int Li = -3; Li <= 3; ++Li) {
+          const int i = Li + si, j = Lj + sj, k = Lk + sk;
+          if (i < h->ni1 || i > h->ni2 || j < h->nj1 || j > h->nj2 || k < h->nk1 || k > h->nk2) continue;
+          const long c = i + h->sy * (long)j + h->sz * (long)k;
+          const float V = volume(i, j, k);
+          cells[c].push_back(SrcContrib{1, n, 0, d[(Li + 3) + 7 * (Lj + 3) + 49 * (Lk + 3)] / V});
+        }
+  }
+  for (auto& kv : surf_of) {
+    const SurfEntry& e = sent[kv.second];
+    smap[kv.first] = kv.second;
+    const long c = kv.first + h->sz * (long)h->nk2;
+    cells[c].push_back(SrcContrib{2, e.n, e.m, 1.0f / e.S});
+  }
+  // device tables
+  std::vector<long> cell;
+  std::vector<int> ptr{0};
+  std::vector<SrcContrib> con;
+  for (auto& kv : cells) {
+    cell.push_back(kv.first);
+    for (auto& s : kv.second) con.push_back(s);
+    ptr.push_back((int)con.size());
+  }
+  h->n_srccell = (int)cell.size();
+  h->n_surf = (int)sent.size();
+  int bad = 0;
+  bad |= dalloc((void**)&h->d_scell, sizeof(long) * cell.size(), cell.data());
+  bad |= dalloc((void**)&h->d_sptr, sizeof(int) * ptr.size(), ptr.data());
+  bad |= dalloc((void**)&h->d_scon, sizeof(SrcContrib) * con.size(), con.data());
+  bad |= dalloc((void**)&h->d_smap, sizeof(int) * smap.size(), smap.data());
+  bad |= dalloc((void**)&h->d_sent, sizeof(SurfEntry) * sent.size(), sent.data());
+  bad |= dalloc((void**)&h->d_ft0, sizeof(float) * h->ft0.size(), h->ft0.data());
+  bad |= dalloc((void**)&h->d_ftime, sizeof(float) * h->ftime.size(), h->ftime.data());
+  bad |= dalloc((void**)&h->d_ffreq, sizeof(float) * h->ffreq.size(), h->ffreq.data());
+  bad |= dalloc((void**)&h->d_F, sizeof(float) * h->F.size(), h->F.data());
+  bad |= dalloc((void**)&h->d_mt0, sizeof(float) * h->mt0.size(), h->mt0.data());
+  bad |= dalloc((void**)&h->d_mtime, sizeof(float) * h->mtime.size(), h->mtime.data());
+  bad |= dalloc((void**)&h->d_mfreq, sizeof(float) * h->mfreq.size(), h->mfreq.data());
+  bad |= dalloc((void**)&h->d_M, sizeof(float) * h->M.size(), h->M.data());
+  return bad ? fail(h, FW_ERR_NOMEM, "build_sources: device allocation failed") : FW_OK;
+}
+
+static int zero_state(fw_handle h) {
+  const long n = 4L * 9 * h->fs;
+  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->set[0] - h->off + 0, n + h->off, 0.f);
+  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((3 * h->ps + 255) / 256)), dim3(256), 0, h->stream, h->d_msurf, 3 * h->ps, 0.f);
+  for (auto& f : h->face)
+    if (f.on) {
+      const long m = 4L * 6 * f.cnt;
+      hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, h->stream, f.d, m, 0.f);
+    }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  h->nrec = 0;
+  for (auto& s : h->snaps) s.nrec = 0;
+  return FW_OK;
+}
+
+int fw_init(fw_handle h) {
+  if (!h) return FW_ERR_ARG;
+  if (!h->have_grid || !h->have_media) return fail(h, FW_ERR_STATE, "fw_init: grid and media must be set first");
+  build_pml(h);
+  // device PML profiles: per axis b1 = 1/B, ad = A + D/B, dd = D/B
+  hipFree(h->d_pml);
+  const long n1 = h->nx + h->ny + h->nz;
+  std::vector<float> prof(3 * n1);
+  auto put = [&](long base, const std::vector<float>& A, const std::vector<float>& B, const std::vector<float>& D) {
+    for (size_t q = 0; q < A.size(); ++q) {
+      prof[base + q] = 1.0f / B[q];
+      prof[n1 + base + q] = A[q] + D[q];
+      prof[2 * n1 + base + q] = D[q];
+    }
+  };
+  put(0, h->Ax, h->Bx, h->Dx);
+  put(h->nx, h->Ay, h->By, h->Dy);
+  put(h->nx + h->ny, h->Az, h->Bz, h->Dz);
+  HIPCHK(hipMalloc(&h->d_pml, sizeof(float) * prof.size()));
+  HIPCHK(hipMemcpy(h->d_pml, prof.data(), sizeof(float) * prof.size(), hipMemcpyHostToDevice));
+  for (int n = 0; n < 6; ++n) {
+    PmlFace& f = h->face[n];
+    hipFree(f.d);
+    f = PmlFace{};
+    const int N = h->cfg.abs_number[n];
+    if (N <= 0) continue;
+    f.on = true; f.n = N;
+    f.lo[0] = h->ni1; f.hi[0] = h->ni2; f.lo[1] = h->nj1; f.hi[1] = h->nj2; f.lo[2] = h->nk1; f.hi[2] = h->nk2;
+    const int ax = n / 2;
+    if (n % 2 == 0) f.hi[ax] = f.lo[ax] + N; else f.lo[ax] = f.hi[ax] - N;
+    f.cnt = 1;
+    for (int d = 0; d < 3; ++d) { f.ex[d] = f.hi[d] - f.lo[d] + 1; f.cnt *= f.ex[d]; }
+    HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt));
+  }
+  int rc = build_sources(h);
+  if (rc) return rc;
+  rc = zero_state(h);
+  if (rc) return rc;
+  h->ready = true;
+  return FW_OK;
+}
+
+int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
+  if (!h || !A || !B || !D) return fail(h, FW_ERR_ARG, "fw_get_pml_profiles: null argument");
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_get_pml_profiles: call fw_init first");
+  std::copy(h->Ax.begin(), h->Ax.end(), A); std::copy(h->Ay.begin(), h->Ay.end(), A + h->nx);
+  std::copy(h->Az.begin(), h->Az.end(), A + h->nx + h->ny);
+  std::copy(h->Bx.begin(), h->Bx.end(), B); std::copy(h->By.begin(), h->By.end(), B + h->nx);
+  std::copy(h->Bz.begin(), h->Bz.end(), B + h->nx + h->ny);
+  std::copy(h->Dx.begin(), h->Dx.end(), D); std::copy(h->Dy.begin(), h->Dy.end(), D + h->nx);
+  std::copy(h->Dz.begin(), h->Dz.end(), D + h->nx + h->ny);
+  return FW_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch tables over the 8 F/B patterns and 3 RK kinds.
+// ---------------------------------------------------------------------------
+typedef void (*StageFn)(StageArgs);
+typedef void (*PmlFn)(PmlArgs);
+
+template <int P, int K>
+static StageFn stage_fn() {
+  return k_stage<(P >> 2) & 1, (P >> 1) & 1, P & 1, K>;
+}
+template <int P, int K, int AX>
+static PmlFn pml_fn() {
+  return k_pml<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, AX>;
+}
+#define ROW3(P, K) stage_fn<P, K>()
+static StageFn stage_table(int p, int kind) {
+  static const StageFn T[8][3] = {
+      {ROW3(0, 0), ROW3(0, 1), ROW3(0, 2)}, {ROW3(1, 0), ROW3(1, 1), ROW3(1, 2)},
+      {ROW3(2, 0), ROW3(2, 1), ROW3(2, 2)}, {ROW3(3, 0), ROW3(3, 1), ROW3(3, 2)},
+      {ROW3(4, 0), ROW3(4, 1), ROW3(4, 2)}, {ROW3(5, 0), ROW3(5, 1), ROW3(5, 2)},
+      {ROW3(6, 0), ROW3(6, 1), ROW3(6, 2)}, {ROW3(7, 0), ROW3(7, 1), ROW3(7, 2)}};
+  return T[p][kind];
+}
+#define PROW(P, K) {pml_fn<P, K, 0>(), pml_fn<P, K, 1>(), pml_fn<P, K, 2>()}
+#define PROW3(P) {PROW(P, 0), PROW(P, 1), PROW(P, 2)}
+static PmlFn pml_table(int p, int kind, int ax) {
+  static const PmlFn T[8][3][3] = {PROW3(0), PROW3(1), PROW3(2), PROW3(3),
+                                   PROW3(4), PROW3(5), PROW3(6), PROW3(7)};
+  return T[p][kind][ax];
+}
+
+extern "C" {
+
+static int launch_stage(fw_handle h, int ntime, int st) {
+  const int* bp = BLOCK_PAT[((ntime % 8) + 8) % 8];
+  const int flip = st & 1;
+  const int xf = flip ? !bp[0] : bp[0], yf = flip ? !bp[1] : bp[1], zf = flip ? !bp[2] : bp[2];
+  const int pat = (xf << 2) | (yf << 1) | zf;
+  const int kind = st == 0 ? 0 : (st == 3 ? 2 : 1);
+  // register rotation: stage in/out sets
+  static const int IN[4] = {0, 1, 2, 1}, OUT[4] = {1, 2, 1, 0};
+  float* win = h->set[IN[st]];
+  float* wout = h->set[OUT[st]];
+  float* A = h->set[0];
+  float* T = h->set[3];
+  static const float RKA[4] = {0.5f, 0.5f, 1.0f, 0.0f};
+  const float RKB[4] = {1.0f / 6.0f, 1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 6.0f};
+  const float tinc = st == 0 ? 0.f : RKA[st - 1];
+  const float ts = ((float)ntime + tinc) * h->cfg.stept;
+  const float a = RKA[st] * h->cfg.stept, b = RKB[st] * h->cfg.stept;
+
+  if (h->freenode) {
+    SurfArgs s{};
+    s.win = win; s.msurf = h->d_msurf; s.smap = h->d_smap; s.ent = h->d_sent;
+    s.ft0 = h->d_ft0; s.ftime = h->d_ftime; s.ffreq = h->d_ffreq; s.F = h->d_F;
+    s.fs = h->fs; s.sy = h->sy; s.sz = h->sz; s.ps = h->ps; s.fblk = (long)h->fntw * h->nforce;
+    s.ni1 = h->ni1; s.ni2 = h->ni2; s.nj1 = h->nj1; s.nj2 = h->nj2; s.nk2 = h->nk2;
+    s.ntw = h->fntw; s.stf_id = h->fstf; s.ntime = ntime; s.save = st == 0; s.ts = ts; s.stept = h->cfg.stept;
+    dim3 blk(64, 4), grd((h->cfg.ni + 63) / 64, (h->cfg.nj + 3) / 4);
+    hipLaunchKernelGGL(k_surface, grd, blk, 0, h->stream, s);
+  }
+  StageArgs g{};
+  g.win = win; g.wout = wout; g.m = A; g.t = T;
+  g.lam = h->lam; g.mu = h->mu; g.rrho = h->rrho;
+  g.xi_x = h->xi_x; g.eta_y = h->eta_y; g.zeta_z = h->zeta_z; g.rz = h->rz; g.rxsin = h->rxsin; g.xcot = h->xcot;
+  g.msurf = h->d_msurf;
+  g.fs = h->fs; g.sy = h->sy; g.sz = h->sz; g.ps = h->ps;
+  g.ni1 = h->ni1; g.ni2 = h->ni2; g.nj1 = h->nj1; g.nj2 = h->nj2; g.nk1 = h->nk1; g.nk2 = h->nk2;
+  g.freenode = h->freenode; g.a = a; g.b = b;
+  const int gx = (h->cfg.ni + BX - 1) / BX, gy = (h->cfg.nj + BY - 1) / BY;
+  int nchunk = std::max(1, std::min(h->cfg.nk / 16, (2048 + gx * gy - 1) / (gx * gy)));
+  g.kchunk = (h->cfg.nk + nchunk - 1) / nchunk;
+  nchunk = (h->cfg.nk + g.kchunk - 1) / g.kchunk;
+  if (h->timing) {
+    hipEventRecord(h->ev[2 * h->nstage], h->stream);
+  }
+  hipLaunchKernelGGL(stage_table(pat, kind), dim3(gx, gy, nchunk), dim3(BX, BY), 0, h->stream, g);
+  if (h->timing) {
+    hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
+    h->nstage++;
+  }
+  // PML faces
+  const long n1 = h->nx + h->ny + h->nz;
+  for (int n = 0; n < 6; ++n) {
+    const PmlFace& f = h->face[n];
+    if (!f.on) continue;
+    const int ax = n / 2;
+    const long axoff = ax == 0 ? 0 : (ax == 1 ? h->nx : h->nx + h->ny);
+    PmlArgs p{};
+    p.win = win; p.wout = wout; p.t = T;
+    p.lam = h->lam; p.mu = h->mu; p.rrho = h->rrho;
+    p.xi_x = h->xi_x; p.eta_y = h->eta_y; p.zeta_z = h->zeta_z; p.rz = h->rz; p.rxsin = h->rxsin; p.xcot = h->xcot;
+    p.pb1 = h->d_pml + axoff; p.pad = h->d_pml + n1 + axoff; p.pdd = h->d_pml + 2 * n1 + axoff;
+    const long set = 6 * f.cnt;
+    p.pin = f.d + IN[st] * set; p.pout = f.d + OUT[st] * set; p.pm = f.d; p.pt = f.d + 3 * set;
+    p.fs = h->fs; p.sy = h->sy; p.sz = h->sz; p.pcnt = f.cnt;
+    p.lo0 = f.lo[0]; p.lo1 = f.lo[1]; p.lo2 = f.lo[2]; p.ex0 = f.ex[0]; p.ex1 = f.ex[1]; p.ex2 = f.ex[2];
+    p.nk2 = h->nk2; p.freenode = h->freenode; p.a = a; p.b = b;
+    hipLaunchKernelGGL(pml_table(pat, kind, ax), dim3((unsigned)((f.cnt + 255) / 256)), dim3(256), 0, h->stream, p);
+  }
+  // point sources
+  if (h->n_srccell > 0) {
+    SrcArgs s{};
+    s.wout = wout; s.t = T; s.lam = h->lam; s.mu = h->mu;
+    s.cell = h->d_scell; s.ptr = h->d_sptr; s.con = h->d_scon;
+    s.ft0 = h->d_ft0; s.ftime = h->d_ftime; s.ffreq = h->d_ffreq; s.F = h->d_F;
+    s.fblk = (long)h->fntw * h->nforce; s.fntw = h->fntw; s.fstf = h->fstf;
+    s.mt0 = h->d_mt0; s.mtime = h->d_mtime; s.mfreq = h->d_mfreq; s.M = h->d_M;
+    s.mblk = (long)h->mntw * h->nmom; s.mntw = h->mntw; s.mstf = h->mstf;
+    s.fs = h->fs; s.ncell = h->n_srccell; s.ntime = ntime; s.kind = kind;
+    s.ts = ts; s.stept = h->cfg.stept; s.a = a; s.b = b;
+    hipLaunchKernelGGL(k_src, dim3((h->n_srccell + 127) / 128), dim3(128), 0, h->stream, s);
+  }
+  return FW_OK;
+}
+
+static int capture_outputs(fw_handle h, int t1) {
+  if (h->npt > 0 && h->d_rec && t1 % h->pt_tinv == 0) {
+    if (h->nrec >= h->max_rec) return fail(h, FW_ERR_STATE, "receiver storage full (max_records)");
+    hipLaunchKernelGGL(k_receivers, dim3((h->npt * 9 + 255) / 256), dim3(256), 0, h->stream, h->set[0], h->fs, h->d_pt,
+                       h->npt, h->d_rec + (size_t)h->nrec * 9 * h->npt);
+    h->nrec++;
+  }
+  for (auto& s : h->snaps) {
+    if (t1 % s.tinv != 0) continue;
+    if (s.nrec >= s.max_records) return fail(h, FW_ERR_STATE, "snapshot storage full (max_records)");
+    SnapArgs a{};
+    a.W = h->set[0]; a.out = s.d + (size_t)s.nrec * s.nf * s.npts;
+    a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = s.npts;
+    a.s0 = s.subs[0]; a.s1 = s.subs[1]; a.s2 = s.subs[2]; a.c0 = s.subc[0]; a.c1 = s.subc[1];
+    a.t0 = s.subt[0]; a.t1 = s.subt[1]; a.t2 = s.subt[2]; a.nf = s.nf;
+    for (int q = 0; q < s.nf; ++q) a.flist[q] = s.flist[q];
+    hipLaunchKernelGGL(k_snapshot, dim3((unsigned)((s.npts + 255) / 256)), dim3(256), 0, h->stream, a);
+    s.nrec++;
+  }
+  return FW_OK;
+}
+
+int fw_step(fw_handle h, int ntime) {
+  if (!h) return FW_ERR_ARG;
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_step: call fw_init first");
+  if (ntime < 0) return fail(h, FW_ERR_ARG, "fw_step: ntime must be >= 0");
+  for (int st = 0; st < 4; ++st) {
+    int rc = launch_stage(h, ntime, st);
+    if (rc) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  return capture_outputs(h, ntime + 1);
+}
+
+int fw_enable_timing(fw_handle h, int on) {
+  if (!h) return FW_ERR_ARG;
+  h->timing = on != 0;
+  return FW_OK;
+}
+
+int fw_run(fw_handle h, int ntime0, int nsteps) {
+  if (!h) return FW_ERR_ARG;
+  if (nsteps < 0) return fail(h, FW_ERR_ARG, "fw_run: nsteps must be >= 0");
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (h->timing) {
+    const size_t need = 2 * (size_t)nsteps * 4;
+    while (h->ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      h->ev.push_back(e);
+    }
+    h->nstage = 0;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    hipEventRecord(t0, h->stream);
+  }
+  for (int n = 0; n < nsteps; ++n) {
+    int rc = fw_step(h, ntime0 + n);
+    if (rc) return rc;
+  }
+  if (h->timing) {
+    hipEventRecord(t1, h->stream);
+    HIPCHK(hipEventSynchronize(t1));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, t0, t1);
+    h->total_ms = ms;
+    double s = 0;
+    for (int q = 0; q < h->nstage; ++q) {
+      float m = 0.f;
+      hipEventElapsedTime(&m, h->ev[2 * q], h->ev[2 * q + 1]);
+      s += m;
+    }
+    h->stage_ms = s;
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+  }
+  return FW_OK;
+}
+
+int fw_get_timing(fw_handle h, double* stage_ms, double* total_ms, int* nst) {
+  if (!h) return FW_ERR_ARG;
+  if (stage_ms) *stage_ms = h->stage_ms;
+  if (total_ms) *total_ms = h->total_ms;
+  if (nst) *nst = h->nstage;
+  return FW_OK;
+}
+
+int fw_sync(fw_handle h) {
+  if (!h) return FW_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+int fw_get_receivers(fw_handle h, float* out, int* nrecords) {
+  if (!h) return FW_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (nrecords) *nrecords = h->nrec;
+  if (out && h->nrec > 0 && h->npt > 0)
+    HIPCHK(hipMemcpy(out, h->d_rec, sizeof(float) * 9 * (size_t)h->npt * h->nrec, hipMemcpyDeviceToHost));
+  return FW_OK;
+}
+
+int fw_get_snapshot(fw_handle h, int id, float* out, int* nrecords) {
+  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_get_snapshot: bad id");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const Snapshot& s = h->snaps[id];
+  if (nrecords) *nrecords = s.nrec;
+  if (out && s.nrec > 0)
+    HIPCHK(hipMemcpy(out, s.d, sizeof(float) * (size_t)s.nf * s.npts * s.nrec, hipMemcpyDeviceToHost));
+  return FW_OK;
+}
+
+int fw_snapshot_device(fw_handle h, int id, float** dptr, int* nrecords, long* npoints) {
+  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_snapshot_device: bad id");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const Snapshot& s = h->snaps[id];
+  if (dptr) *dptr = s.d;
+  if (nrecords) *nrecords = s.nrec;
+  if (npoints) *npoints = s.npts;
+  return FW_OK;
+}
+
+int fw_get_wavefield(fw_handle h, int f, float* out) {
+  if (!h || f < 0 || f >= 9 || !out) return fail(h, FW_ERR_ARG, "fw_get_wavefield: bad argument");
+  return download_volume(h, out, h->set[0] + (size_t)f * h->fs);
+}
+
+int fw_set_wavefield(fw_handle h, int f, const float* in) {
+  if (!h || f < 0 || f >= 9 || !in) return fail(h, FW_ERR_ARG, "fw_set_wavefield: bad argument");
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_set_wavefield: call fw_init first");
+  int rc = upload_volume(h, h->set[0] + (size_t)f * h->fs, in);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+/*
+ * fwant.h -- C ABI of the MI355X-native FWANT solver and sensitivity-kernel
+ * library (libfwant.so).  Plain pointers and sizes only; every function
+ * returns FW_OK (0) or a negative status, with a message in fw_last_error().
+ *
+ * Conventions shared with the reference Fortran code:
+ *  - 3-D arrays are column-major (i fastest) over the ghosted index space
+ *    nx = ni+6, ny = nj+6, nz = nk+6 (code/srcF/mod_para.F90:79-82).
+ *  - Grid indices passed through this ABI are the reference's 1-based local
+ *    indices (ghost offset included), exactly as stored in source_mpi*.nc
+ *    and station_mpi*.nc.
+ *  - Field order: Vx Vy Vz Txx Tyy Tzz Txy Txz Tyz (seismogram variable
+ *    order, code/srcF/mod_io.F90:436-444).
+ *
+ * One host thread per handle.  Each handle owns one HIP stream; calls that
+ * launch work are stream-ordered and return before the work completes
+ * unless stated; fw_sync() waits.  Host output buffers are owned by the
+ * caller.
+ */
+#ifndef FWANT_H
+#define FWANT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FW_OK 0
+#define FW_ERR_ARG (-1)
+#define FW_ERR_HIP (-2)
+#define FW_ERR_STATE (-3)
+#define FW_ERR_NOMEM (-4)
+
+enum { FW_VX = 0, FW_VY, FW_VZ, FW_TXX, FW_TYY, FW_TZZ, FW_TXY, FW_TXZ, FW_TYZ, FW_NFIELD };
+
+/* source-time-function ids, code/srcF/mod_src.F90:44-52 */
+enum {
+  FW_SVF_BELL = 100, FW_SVF_TRIANGLE = 110, FW_SVF_GAUSS = 120, FW_STF_RICKER = 200,
+  FW_STF_BSHIFT = 210, FW_STF_GAUSS = 220, FW_STF_BELL = 230, FW_STF_STEP = 240, FW_STF_DELTA = 250
+};
+
+typedef struct fw_solver_s *fw_handle;
+
+/* Solver configuration: the SeisFD3D.conf keys that shape the solve.
+ *   ni nj nk stept         code/srcF/mod_para.F90:75-84
+ *   freenode               top rank owns the free surface, mod_mpi.F90:206-216
+ *   abs_number, abs_velocity, CFS_bmax, CFS_amax (x1 x2 y1 y2 z1 z2)
+ *                          code/srcF/mod_abs_npml.F90:160-169
+ * A z2 absorbing layer clears freenode, as abs_init does (mod_abs_npml.F90:182-184). */
+typedef struct fw_config {
+  int ni, nj, nk;
+  float stept;
+  int freenode;
+  int abs_number[6];
+  float abs_velocity[6];
+  float cfs_bmax[6];
+  float cfs_amax[6];
+} fw_config;
+
+/* Replaces macdrp_init + the module allocations (code/srcF/mod_macdrp.F90:124-218)
+ * and the para/grid/media allocations of the driver (seis3d_wave_8op_p3.F90:40-66). */
+int fw_create(const fw_config *cfg, fw_handle *out);
+/* Replaces macdrp_destroy / abs_destroy / src_destroy (mod_macdrp.F90:219-227). */
+int fw_destroy(fw_handle h);
+/* Last error message of this handle (h may be NULL: last error of fw_create). */
+const char *fw_last_error(fw_handle h);
+/* Ghosted extents and the device pitch used for 3-D arrays. */
+int fw_dims(fw_handle h, int *nx, int *ny, int *nz, int *freenode);
+
+/* Replaces grid_coord_import + grid_metric_import (code/srcF/mod_grid.F90:96-114):
+ * 1-D coordinate and metric vectors of lengths nx (x, xsin, xcot, xi_x),
+ * ny (y, eta_y), nz (z, zeta_z).  Copied. */
+int fw_set_grid(fw_handle h, const float *x, const float *y, const float *z, const float *xsin,
+                const float *xcot, const float *xi_x, const float *eta_y, const float *zeta_z);
+/* Replaces seis3d_metric (code/srcF/seis3d_metric.F90:82-112): builds xsin, xcot,
+ * xi_x, eta_y, zeta_z on the GPU in FP64 from double coordinates and installs
+ * them (stored FP32).  If the out_* pointers are non-NULL the FP32 metric is
+ * copied back (lengths as in fw_set_grid). */
+int fw_build_metric(fw_handle h, const double *x, const double *y, const double *z, float *out_xsin,
+                    float *out_xcot, float *out_xi_x, float *out_eta_y, float *out_zeta_z);
+/* Replaces media_import (code/srcF/mod_media.F90:241-255): ghosted nx*ny*nz arrays. */
+int fw_set_media(fw_handle h, const float *lambda, const float *mu, const float *rho);
+
+/* Replaces src_import for force sources (code/srcF/mod_src.F90:168-185):
+ *   indx[3*n] (1-based local), shift[3*n], t0[n], stf_time[ntw], stf_freq[ntw],
+ *   Fx/Fy/Fz[ntw*n] (window index fastest). */
+int fw_set_force(fw_handle h, int n, int ntw, int stf_id, const int *indx, const float *shift,
+                 const float *t0, const float *stf_time, const float *stf_freq, const float *Fx,
+                 const float *Fy, const float *Fz);
+/* Replaces src_import for moment sources (mod_src.F90:150-167):
+ *   M6[6*ntw*n] = Mxx, Myy, Mzz, Mxy, Mxz, Myz blocks, each [ntw*n]. */
+int fw_set_moment(fw_handle h, int n, int ntw, int stf_id, const int *indx, const float *shift,
+                  const float *t0, const float *stf_time, const float *stf_freq, const float *M6);
+/* Replaces io_pt_import + io_seismo_init (code/srcF/mod_io.F90:160-176, 430-459):
+ * n receivers at 1-based local indices indx[3*n], sampled every tinv steps;
+ * device storage for max_records records of 9 values each is reserved. */
+int fw_set_receivers(fw_handle h, int n, int tinv, const int *indx, int max_records);
+/* Replaces io_snap_read/locate + the snapshot part of io_wave_export
+ * (code/srcF/mod_io.F90:180-226, 505-645): a strided sub-volume subs/subc/subt
+ * (1-based local start, counts, strides) of the fields in field_mask
+ * (bit f = field f) is captured every tinv steps into device storage for
+ * max_records records.  Returns its id in *snap_id. */
+int fw_add_snapshot(fw_handle h, const int *subs, const int *subc, const int *subt, int tinv,
+                    unsigned field_mask, int max_records, int *snap_id);
+
+/* Replaces abs_init (code/srcF/mod_abs_npml.F90:135-300) and finishes set-up:
+ * PML profiles, source tables, zero wavefield.  Must follow the set_* calls. */
+int fw_init(fw_handle h);
+/* Host copies of the PML profiles (A, B and D/B as abs_init stores them),
+ * each of length nx+ny+nz (x part, then y, then z). */
+int fw_get_pml_profiles(fw_handle h, float *A, float *B, float *D);
+
+/* One time step of seis3d_wave_8op_p3.F90:97-157: the 4 RK stages with the
+ * operator pattern of block (ntime mod 8) (seis3d_wave_8op_p3.F90:82-591),
+ * then receiver and snapshot capture for step ntime+1.  Asynchronous. */
+int fw_step(fw_handle h, int ntime);
+/* nsteps consecutive steps starting at ntime0. Asynchronous. */
+int fw_run(fw_handle h, int ntime0, int nsteps);
+/* Steps the reference driver runs for a given nt: 8*ceil((nt+1)/8)
+ * (the exit test sits at the top of each 8-step block, seis3d_wave_8op_p3.F90:92-95). */
+int fw_reference_steps(int nt);
+/* Wait for all work of the handle. */
+int fw_sync(fw_handle h);
+
+/* Receiver records captured so far: out[nrec][npt][9]; *nrecords receives the
+ * record count (io_seismo_put layout, mod_io.F90:461-497).  Synchronises. */
+int fw_get_receivers(fw_handle h, float *out, int *nrecords);
+/* Snapshot records: out[nrec][nfield_in_mask][subc0*subc1*subc2] (point index
+ * i fastest; the fields of the mask in increasing field index order).  Synchronises.  fw_snapshot_device returns the device pointer of
+ * the same storage (valid until fw_destroy) for zero-copy consumers. */
+int fw_get_snapshot(fw_handle h, int snap_id, float *out, int *nrecords);
+int fw_snapshot_device(fw_handle h, int snap_id, float **dptr, int *nrecords, long *npoints);
+/* Wavefield state (between steps): ghosted nx*ny*nz host arrays.  Replaces
+ * io_rest_export / io_rest_import's array transfer (mod_io.F90:342-427). */
+int fw_get_wavefield(fw_handle h, int field, float *out);
+int fw_set_wavefield(fw_handle h, int field, const float *in);
+
+/* Per-kernel timing of the last fw_run (milliseconds of the main stage
+ * kernels summed, measured with HIP events on the handle's stream). */
+int fw_enable_timing(fw_handle h, int on);
+int fw_get_timing(fw_handle h, double *stage_ms, double *total_ms, int *nstage_launches);
+
+/* ---------------------------------------------------------------------------
+ * Sensitivity-kernel time integral (code/srcF.kernel/SI_ker_sta.F90).
+ * All arrays are DEVICE pointers.  A "series" array holds component c,
+ * sample n (0-based), point p at base[c*cs + n*ts + p].  stream may be NULL
+ * (default stream).
+ * ------------------------------------------------------------------------- */
+/* stress2strain, SI_ker_sta.F90:1185-1211 (in place, 6 components xx yy zz xy xz yz) */
+int fwk_stress2strain(float *E, long cs, long ts, long npt, int knt, const float *lam, const float *mu,
+                      void *stream);
+/* sgt2scatter3 for one receiver component, SI_ker_sta.F90:1257-1337:
+ * outA (alpha series) and outB (beta series), [knt][npt].  FP32 products,
+ * FP64 accumulation. */
+int fwk_scatter(const float *R, long rcs, long rts, const float *S, long scs, long sts, long npt, int knt,
+                float dt, const float *lam, const float *mu, float *outA, float *outB, void *stream);
+/* scatter_filter (mode 1, "P1") / scatter_filtfilt (mode 2, "P2"),
+ * SI_ker_sta.F90:1433-1475; b, a are HOST arrays of nod1 doubles. w0, w: [knt][npt]. */
+int fwk_filter(int mode, long npt, int knt, int nod1, const double *b, const double *a, const float *w0,
+               float *w, void *stream);
+/* scatter2kernel, SI_ker_sta.F90:1375-1431; V, U are HOST arrays of knt floats;
+ * nt1, nt2 1-based inclusive.  Outputs Kap Kaq Kbp Kbq [npt]. */
+int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, const float *V,
+                       const float *U, const float *Sa, const float *Sb, float *Kap, float *Kaq, float *Kbp,
+                       float *Kbq, void *stream);
+/* scatter_rotate, SI_ker_sta.F90:1354-1373 (rot: HOST 3x3 column-major). In place. */
+int fwk_rotate(long n, float *X, float *Y, float *Z, const float *rot, void *stream);
+/* Last error of the fwk_* family. */
+const char *fwk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
