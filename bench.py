@@ -46,7 +46,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic", type=str, default=None,
                     help="JSON file with measured HBM bytes per stage-kernel launch")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(4):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 4 copies of 1 GiB (read 1 GiB + write 1 GiB each)")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(virtual sources shard across GPUs with no exchange; see the sharding design)", file=sys.stderr)

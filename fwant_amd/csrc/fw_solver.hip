@@ -23,6 +23,7 @@
 //               (mod_abs_npml.F90:1149-2123, 738-1105)
 //   k_src       point-source terms (mod_src.F90:207-414), RK fix-up
 #include "fw_common.h"
+#include "fw_stage.h"
 #include "../../include/fwant.h"
 
 #include <algorithm>
@@ -47,178 +48,10 @@ thread_local std::string g_create_err;
     }                                                                             \
   } while (0)
 
-constexpr int BX = 64, BY = 4;
-
-struct StageArgs {
-  const float* __restrict__ win;
-  float* __restrict__ wout;
-  const float* __restrict__ m;
-  float* __restrict__ t;
-  const float* __restrict__ lam;
-  const float* __restrict__ mu;
-  const float* __restrict__ rrho;
-  const float* __restrict__ xi_x;
-  const float* __restrict__ eta_y;
-  const float* __restrict__ zeta_z;
-  const float* __restrict__ rz;
-  const float* __restrict__ rxsin;
-  const float* __restrict__ xcot;
-  const float* __restrict__ msurf;  // saved Tzz, Txz, Tyz at k = nk2 (3 planes)
-  long fs, sy, sz, ps;               // field stride, j stride, k stride, plane-array stride
-  int ni1, ni2, nj1, nj2, nk1, nk2;
-  int kchunk, freenode;
-  float a, b;
-};
-
-// ---------------------------------------------------------------------------
-// Fused stage kernel: RHS of Lx?_Ly?_Lz? (mod_macdrp.F90:790-962) for one
-// (i, j) column per thread, marching k with a 7-slot register window for the
-// z-derivatives of Tzz Txz Tyz Vx Vy Vz, followed by the RK update.
-//   KIND 0: Wout = m + a h, T = m + b h (m = Win)     RK_beg
-//   KIND 1: Wout = m + a h, T = T + b h               RK_inn
-//   KIND 2: Wout = T + b h                            RK_fin
-// The free-surface traction condition at k = nk2 is applied with zero source
-// terms; k_src adds the source traction-rate part.
-// ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int KIND>
-__global__ __launch_bounds__(BX* BY) void k_stage(StageArgs g) {
-  const int i = g.ni1 + blockIdx.x * BX + threadIdx.x;
-  const int j = g.nj1 + blockIdx.y * BY + threadIdx.y;
-  if (i > g.ni2 || j > g.nj2) return;
-  const int k0 = g.nk1 + blockIdx.z * g.kchunk;
-  const int k1 = min(k0 + g.kchunk - 1, g.nk2);
-  const long fs = g.fs, sy = g.sy, sz = g.sz;
-  const float xi = g.xi_x[i], et = g.eta_y[j], rxs = g.rxsin[i], xc = g.xcot[i];
-  const float* __restrict__ W = g.win;
-
-  constexpr int ZFLD[6] = {TZZ, TXZ, TYZ, VX, VY, VZ};
-  float w[6][7];
-  long c = i + sy * j + sz * (long)k0;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-#pragma unroll
-    for (int o = 0; o < 7; ++o) w[q][o] = 0.f;
-#pragma unroll
-    for (int o = (ZF ? -1 : -3); o <= (ZF ? 2 : 0); ++o) w[q][o + 3] = W[ZFLD[q] * fs + c + o * sz];
-  }
-
-  for (int k = k0; k <= k1; ++k, c += sz) {
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      if constexpr (ZF)
-        w[q][6] = W[ZFLD[q] * fs + c + 3 * sz];
-      else
-        w[q][4] = W[ZFLD[q] * fs + c + sz];
-    }
-    const float lam = g.lam[c], miu = g.mu[c], rr = g.rrho[c];
-    const float lam2mu = lam + 2.0f * miu;
-    const float ze = g.zeta_z[k], rzk = g.rz[k];
-
-    const float* pTxx = W + TXX * fs + c;
-    const float* pTyy = W + TYY * fs + c;
-    const float* pTxy = W + TXY * fs + c;
-    const float* pTxz = W + TXZ * fs + c;
-    const float* pTyz = W + TYZ * fs + c;
-    const float* pVx = W + VX * fs + c;
-    const float* pVy = W + VY * fs + c;
-    const float* pVz = W + VZ * fs + c;
-
-    float DxTxx = dop<XF>(pTxx, 1) * xi;
-    float DxTxy = dop<XF>(pTxy, 1) * xi;
-    float DxTxz = dop<XF>(pTxz, 1) * xi;
-    const float DxVx = dop<XF>(pVx, 1) * xi;
-    const float DxVy = dop<XF>(pVy, 1) * xi;
-    const float DxVz = dop<XF>(pVz, 1) * xi;
-    const float DyTyy = dop<YF>(pTyy, sy) * et;
-    float DyTxy = dop<YF>(pTxy, sy) * et;
-    float DyTyz = dop<YF>(pTyz, sy) * et;
-    const float DyVx = dop<YF>(pVx, sy) * et;
-    const float DyVy = dop<YF>(pVy, sy) * et;
-    const float DyVz = dop<YF>(pVz, sy) * et;
-    const float DzTzz = dopw<ZF>(w[0]) * ze;
-    float DzTxz = dopw<ZF>(w[1]) * ze;
-    float DzTyz = dopw<ZF>(w[2]) * ze;
-    float DzVx, DzVy, DzVz;
-    if (g.freenode && k == g.nk2 - 1) {  // CondFreeVLOW, mod_macdrp.F90:872-886
-      DzVx = d22w<ZF>(w[3]) * ze;
-      DzVy = d22w<ZF>(w[4]) * ze;
-      DzVz = d22w<ZF>(w[5]) * ze;
-    } else if (g.freenode && k == g.nk2 - 2) {  // mod_macdrp.F90:887-900
-      DzVx = d24w<ZF>(w[3]) * ze;
-      DzVy = d24w<ZF>(w[4]) * ze;
-      DzVz = d24w<ZF>(w[5]) * ze;
-    } else {
-      DzVx = dopw<ZF>(w[3]) * ze;
-      DzVy = dopw<ZF>(w[4]) * ze;
-      DzVz = dopw<ZF>(w[5]) * ze;
-    }
-    if (miu <= SEIS_ZERO) {  // fluid cells, mod_macdrp.F90:917-926
-      DxTxy = 0.f; DxTxz = 0.f; DyTxy = 0.f; DyTyz = 0.f; DzTxz = 0.f; DzTyz = 0.f;
-    }
-    const float Txx = pTxx[0], Tyy = pTyy[0], Txy = pTxy[0];
-    const float Tzz = w[0][3], Txz = w[1][3], Tyz = w[2][3];
-    const float Vx = w[3][3], Vy = w[4][3], Vz = w[5][3];
-
-    float h[9];
-    // momentum with the spherical geometric terms, mod_macdrp.F90:928-933
-    h[VX] = rr * (DxTxx * rzk + DyTxy * rzk * rxs + DzTxz + (3.0f * Txz + Txx * xc - Tyy * xc) * rzk);
-    h[VY] = rr * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
-    h[VZ] = rr * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
-    // strain rates, mod_macdrp.F90:935-941
-    const float E11 = (DxVx + Vz) * rzk;
-    const float E22 = (Vx * xc + DyVy * rxs + Vz) * rzk;
-    const float E12 = (DxVy + DyVx * rxs - Vy * xc) * rzk * 0.5f;
-    float E33 = DzVz;
-    float E13 = (DxVz * rzk + DzVx - Vx * rzk) * 0.5f;
-    float E23 = (DyVz * rzk * rxs + DzVy - Vy * rzk) * 0.5f;
-    const bool surf = g.freenode && k == g.nk2;
-    if (surf) {  // traction condition, mod_macdrp.F90:942-950 (source part in k_src)
-      E33 = -lam / lam2mu * (E11 + E22);
-      E13 = 0.f;
-      E23 = 0.f;
-    }
-    // Hooke, mod_macdrp.F90:952-957
-    h[TXX] = lam2mu * E11 + lam * (E22 + E33);
-    h[TYY] = lam2mu * E22 + lam * (E11 + E33);
-    h[TZZ] = lam2mu * E33 + lam * (E11 + E22);
-    h[TXY] = 2.0f * miu * E12;
-    h[TXZ] = 2.0f * miu * E13;
-    h[TYZ] = 2.0f * miu * E23;
-
-    if constexpr (KIND == 2) {
-#pragma unroll
-      for (int f = 0; f < 9; ++f) g.wout[f * fs + c] = g.t[f * fs + c] + g.b * h[f];
-    } else {
-      float mc[9];
-      if constexpr (KIND == 0) {
-        mc[VX] = Vx; mc[VY] = Vy; mc[VZ] = Vz; mc[TXX] = Txx; mc[TYY] = Tyy; mc[TXY] = Txy;
-        mc[TZZ] = Tzz; mc[TXZ] = Txz; mc[TYZ] = Tyz;
-      } else {
-#pragma unroll
-        for (int f = 0; f < 9; ++f) mc[f] = g.m[f * fs + c];
-      }
-      if (surf) {  // pre-image surface stresses are the RK "m" values there
-        const long c2 = i + sy * j;
-        mc[TZZ] = g.msurf[c2];
-        mc[TXZ] = g.msurf[g.ps + c2];
-        mc[TYZ] = g.msurf[2 * g.ps + c2];
-      }
-#pragma unroll
-      for (int f = 0; f < 9; ++f) {
-        g.wout[f * fs + c] = mc[f] + g.a * h[f];
-        if constexpr (KIND == 0)
-          g.t[f * fs + c] = mc[f] + g.b * h[f];
-        else
-          g.t[f * fs + c] = g.t[f * fs + c] + g.b * h[f];
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-#pragma unroll
-      for (int o = 0; o < 6; ++o) w[q][o] = w[q][o + 1];
-    }
-  }
-}
+// stage-kernel tile shapes (threads 64 x BYT, RY rows per thread): selected
+// per handle (FWANT_TILE or the autotuner), all 16-row tiles
+constexpr int NTILECFG = 2;
+constexpr int TILE_RY[NTILECFG] = {2, 4};  // 8-row and 16-row tiles of 256 threads
 
 // ---------------------------------------------------------------------------
 // Free-surface stress imaging (free_timg, mod_macdrp.F90:3414-3436) on the
@@ -638,6 +471,8 @@ struct fw_solver_s {
   float* d_1d = nullptr;
   float *xi_x, *eta_y, *zeta_z, *rz, *rxsin, *xcot;
   float* d_msurf = nullptr;
+  int* d_fluidcol = nullptr;  // per (i, j) column: 1 if any cell has mu <= SEIS_ZERO
+  int tilecfg = 0;            // stage-kernel tile shape index
   // host copies
   std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
   std::vector<float> hlam, hmu, hrho;
@@ -705,12 +540,13 @@ int fw_create(const fw_config* cfg, fw_handle* out) {
   h = new fw_solver_s();
   h->cfg = *cfg;
   h->nx = cfg->ni + 6; h->ny = cfg->nj + 6; h->nz = cfg->nk + 6;
-  h->px = (h->nx + 15) / 16 * 16;
+  h->px = (h->nx + 31) / 32 * 32;
   h->sy = h->px; h->sz = (long)h->px * h->ny; h->ps = h->sz;
-  h->off = 13;  // (off + ni1) % 16 == 0 with ni1 = 3
-  h->fs = h->sz * h->nz + 16;
+  h->off = 29;  // (off + ni1) % 32 == 0 with ni1 = 3: interior rows start on a 128-byte line
+  h->fs = h->sz * h->nz + 32;
   h->ni1 = 3; h->ni2 = 3 + cfg->ni - 1; h->nj1 = 3; h->nj2 = 3 + cfg->nj - 1; h->nk1 = 3; h->nk2 = 3 + cfg->nk - 1;
   h->freenode = cfg->freenode ? 1 : 0;
+  if (const char* tv = getenv("FWANT_TILE")) h->tilecfg = std::max(0, std::min(NTILECFG - 1, atoi(tv)));
   if (cfg->abs_number[5] > 0) h->freenode = 0;  // mod_abs_npml.F90:182-184
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete h; return fail(nullptr, FW_ERR_HIP, "hipStreamCreate failed"); }
@@ -735,6 +571,7 @@ int fw_create(const fw_config* cfg, fw_handle* out) {
 int fw_destroy(fw_handle h) {
   if (!h) return FW_OK;
   if (h->stream) hipStreamSynchronize(h->stream);
+  hipFree(h->d_fluidcol);
   hipFree(h->d_sets); hipFree(h->d_media); hipFree(h->d_1d); hipFree(h->d_msurf); hipFree(h->d_pml);
   for (auto& f : h->face) hipFree(f.d);
   for (void* p : h->d_src_allocs) hipFree(p);
@@ -846,6 +683,13 @@ int fw_set_media(fw_handle h, const float* lam, const float* mu, const float* rh
   const size_t n = (size_t)h->nx * h->ny * h->nz;
   h->hlam.assign(lam, lam + n); h->hmu.assign(mu, mu + n); h->hrho.assign(rho, rho + n);
   std::vector<float> rr(n);
+  std::vector<int> fluid((size_t)h->ps, 0);
+  for (int k = 0; k < h->nz; ++k)
+    for (int j = 0; j < h->ny; ++j)
+      for (int i = 0; i < h->nx; ++i)
+        if (mu[i + (size_t)h->nx * (j + (size_t)h->ny * k)] <= SEIS_ZERO) fluid[i + h->sy * j] = 1;
+  if (!h->d_fluidcol) HIPCHK(hipMalloc(&h->d_fluidcol, sizeof(int) * h->ps));
+  HIPCHK(hipMemcpy(h->d_fluidcol, fluid.data(), sizeof(int) * h->ps, hipMemcpyHostToDevice));
   for (size_t q = 0; q < n; ++q) {
     if (!(rho[q] > 0.f)) return fail(h, FW_ERR_ARG, "fw_set_media: rho must be > 0 everywhere");
     rr[q] = 1.0f / rho[q];
@@ -1198,22 +1042,29 @@ int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
 typedef void (*StageFn)(StageArgs);
 typedef void (*PmlFn)(PmlArgs);
 
-template <int P, int K>
-static StageFn stage_fn() {
-  return k_stage<(P >> 2) & 1, (P >> 1) & 1, P & 1, K>;
+template <int P, int K, int C>
+static StageFn vel_fn() {
+  return k_vel<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C]>;
+}
+template <int P, int K, int C>
+static StageFn str_fn() {
+  return k_str<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C]>;
 }
 template <int P, int K, int AX>
 static PmlFn pml_fn() {
   return k_pml<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, AX>;
 }
-#define ROW3(P, K) stage_fn<P, K>()
-static StageFn stage_table(int p, int kind) {
-  static const StageFn T[8][3] = {
-      {ROW3(0, 0), ROW3(0, 1), ROW3(0, 2)}, {ROW3(1, 0), ROW3(1, 1), ROW3(1, 2)},
-      {ROW3(2, 0), ROW3(2, 1), ROW3(2, 2)}, {ROW3(3, 0), ROW3(3, 1), ROW3(3, 2)},
-      {ROW3(4, 0), ROW3(4, 1), ROW3(4, 2)}, {ROW3(5, 0), ROW3(5, 1), ROW3(5, 2)},
-      {ROW3(6, 0), ROW3(6, 1), ROW3(6, 2)}, {ROW3(7, 0), ROW3(7, 1), ROW3(7, 2)}};
-  return T[p][kind];
+#define VROW(P, C) {vel_fn<P, 0, C>(), vel_fn<P, 1, C>(), vel_fn<P, 2, C>()}
+#define SROW(P, C) {str_fn<P, 0, C>(), str_fn<P, 1, C>(), str_fn<P, 2, C>()}
+#define VTAB(C) {VROW(0, C), VROW(1, C), VROW(2, C), VROW(3, C), VROW(4, C), VROW(5, C), VROW(6, C), VROW(7, C)}
+#define STAB(C) {SROW(0, C), SROW(1, C), SROW(2, C), SROW(3, C), SROW(4, C), SROW(5, C), SROW(6, C), SROW(7, C)}
+static StageFn vel_table(int c, int p, int kind) {
+  static const StageFn T[NTILECFG][8][3] = {VTAB(0), VTAB(1)};
+  return T[c][p][kind];
+}
+static StageFn str_table(int c, int p, int kind) {
+  static const StageFn T[NTILECFG][8][3] = {STAB(0), STAB(1)};
+  return T[c][p][kind];
 }
 #define PROW(P, K) {pml_fn<P, K, 0>(), pml_fn<P, K, 1>(), pml_fn<P, K, 2>()}
 #define PROW3(P) {PROW(P, 0), PROW(P, 1), PROW(P, 2)}
@@ -1261,14 +1112,18 @@ static int launch_stage(fw_handle h, int ntime, int st) {
   g.fs = h->fs; g.sy = h->sy; g.sz = h->sz; g.ps = h->ps;
   g.ni1 = h->ni1; g.ni2 = h->ni2; g.nj1 = h->nj1; g.nj2 = h->nj2; g.nk1 = h->nk1; g.nk2 = h->nk2;
   g.freenode = h->freenode; g.a = a; g.b = b;
-  const int gx = (h->cfg.ni + BX - 1) / BX, gy = (h->cfg.nj + BY - 1) / BY;
-  int nchunk = std::max(1, std::min(h->cfg.nk / 16, (2048 + gx * gy - 1) / (gx * gy)));
+  g.nx = h->nx; g.ny = h->ny; g.fluidcol = h->d_fluidcol;
+  g.ntx = (h->cfg.ni + TX - 1) / TX;
+  const int tc = h->tilecfg, tyr = BYT * TILE_RY[tc];
+  g.nty = (h->cfg.nj + tyr - 1) / tyr;
+  const int ntile = g.ntx * g.nty;
+  int nchunk = std::max(1, std::min(h->cfg.nk / 16, (1024 + ntile - 1) / ntile));
   g.kchunk = (h->cfg.nk + nchunk - 1) / nchunk;
-  nchunk = (h->cfg.nk + g.kchunk - 1) / g.kchunk;
-  if (h->timing) {
-    hipEventRecord(h->ev[2 * h->nstage], h->stream);
-  }
-  hipLaunchKernelGGL(stage_table(pat, kind), dim3(gx, gy, nchunk), dim3(BX, BY), 0, h->stream, g);
+  g.nch = (h->cfg.nk + g.kchunk - 1) / g.kchunk;
+  const dim3 sgrid((unsigned)(ntile * g.nch)), sblk(TX, BYT);
+  if (h->timing) hipEventRecord(h->ev[2 * h->nstage], h->stream);
+  hipLaunchKernelGGL(vel_table(tc, pat, kind), sgrid, sblk, 0, h->stream, g);
+  hipLaunchKernelGGL(str_table(tc, pat, kind), sgrid, sblk, 0, h->stream, g);
   if (h->timing) {
     hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
     h->nstage++;
