@@ -50,13 +50,14 @@ def main():
                     help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
     args = ap.parse_args()
     if args.copy_calibration:
-        import torch
-        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
+        import numpy as np
+        from fwant_amd._lib import DeviceArray, device_sync
+        a = DeviceArray.from_host(np.ones(1 << 28, np.float32))
         for _ in range(4):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        log("copy calibration: 4 copies of 1 GiB (read 1 GiB + write 1 GiB each)")
+            b = a.copy()
+            b.free()
+        device_sync()
+        log("copy calibration: 4 device copies of 1 GiB (read 1 GiB + write 1 GiB each)")
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -71,19 +72,15 @@ def main():
     log(f"model {m.name} {m.ni}x{m.nj}x{m.nk} built in {time.time() - t:.1f}s")
     s = Solver(m, max_records=args.steps + args.warmup + 64)
     ncell = m.ni * m.nj * m.nk
-    try:
-        import torch
-        sync_torch = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync_torch = lambda: None  # noqa: E731
+    from fwant_amd._lib import device_sync
 
     s.run(args.warmup, ntime0=0)
     s.sync()
-    sync_torch()
+    device_sync()
     t0 = time.perf_counter()
     s.run(args.steps, ntime0=args.warmup)
     s.sync()
-    sync_torch()
+    device_sync()
     el = time.perf_counter() - t0
     value = ncell * args.steps / el
     log(f"timed {args.steps} steps in {el:.3f}s -> {value / 1e9:.3f} G cell-updates/s")

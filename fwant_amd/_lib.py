@@ -6,6 +6,8 @@ CPU fallback in the product path.
 import ctypes as C
 import os
 
+import numpy as np
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfwant.so")
 
@@ -21,7 +23,8 @@ SYMBOLS = (
     "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_get_wavefield",
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
-    "fwk_last_error",
+    "fwk_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
+    "fw_memcpy_d2d", "fw_device_sync",
 )
 
 
@@ -80,6 +83,12 @@ def _sig(lib):
                                          vp, vp, vp, vp, vp, vp, vp]),
         "fwk_rotate": (C.c_int, [C.c_long, vp, vp, vp, fp, vp]),
         "fwk_last_error": (C.c_char_p, []),
+        "fw_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
+        "fw_device_free": (C.c_int, [vp]),
+        "fw_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),
+        "fw_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
+        "fw_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
+        "fw_device_sync": (C.c_int, []),
     }
     for name, (res, args) in S.items():
         f = getattr(lib, name)
@@ -127,3 +136,51 @@ def dptr(a):
 
 def iptr(a):
     return a.ctypes.data_as(ip)
+
+
+class DeviceArray:
+    """A float32/float64 array in HBM owned by this object (allocated through
+    the library's HIP runtime)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        kcheck(lib().fw_device_alloc(self.nbytes, C.byref(p)), "fw_device_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        kcheck(lib().fw_memcpy_h2d(C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), d.nbytes), "fw_memcpy_h2d")
+        return d
+
+    def to_host(self):
+        out = np.empty(self.shape, self.dtype)
+        kcheck(lib().fw_memcpy_d2h(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.nbytes), "fw_memcpy_d2h")
+        return out
+
+    def copy(self):
+        d = DeviceArray(self.shape, self.dtype)
+        kcheck(lib().fw_memcpy_d2d(C.c_void_p(d.ptr), C.c_void_p(self.ptr), self.nbytes), "fw_memcpy_d2d")
+        return d
+
+    def numel(self):
+        return int(np.prod(self.shape))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().fw_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_sync():
+    kcheck(lib().fw_device_sync(), "fw_device_sync")

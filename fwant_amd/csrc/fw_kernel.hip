@@ -217,6 +217,34 @@ extern "C" {
 
 const char* fwk_last_error(void) { return g_kerr.c_str(); }
 
+// device memory helpers for hosts without their own HIP allocator
+int fw_device_alloc(size_t bytes, void** dptr) {
+  if (!dptr) return kfail(FW_ERR_ARG, "fw_device_alloc: null argument");
+  *dptr = nullptr;
+  KCHK(hipMalloc(dptr, bytes ? bytes : 16));
+  return FW_OK;
+}
+int fw_device_free(void* dptr) {
+  KCHK(hipFree(dptr));
+  return FW_OK;
+}
+int fw_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  KCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return FW_OK;
+}
+int fw_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  KCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return FW_OK;
+}
+int fw_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  KCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return FW_OK;
+}
+int fw_device_sync(void) {
+  KCHK(hipDeviceSynchronize());
+  return FW_OK;
+}
+
 int fwk_stress2strain(float* E, long cs, long ts, long npt, int knt, const float* lam, const float* mu, void* stream) {
   if (!E || !lam || !mu || npt <= 0 || knt <= 0) return kfail(FW_ERR_ARG, "fwk_stress2strain: bad argument");
   const unsigned gy = (unsigned)std::min(knt, 64);

@@ -16,11 +16,12 @@
 // Kernels per RK stage:
 //   k_surface   free-surface stress imaging (mod_macdrp.F90:3414-3436),
 //               saving the pre-image surface stresses at stage 1
-//   k_stage     fused operator sweep + RK update (mod_macdrp.F90:790-962,
-//               291-393); z-marching columns, register window in z
-//   k_pml       CFS-PML correction of one face slab, applied as an RK
-//               fix-up on the stage outputs, plus the auxiliary RK update
-//               (mod_abs_npml.F90:1149-2123, 738-1105)
+//   k_vel       velocity half of the operator sweep + CFS-PML (stress
+//               derivative auxiliaries) + RK update           (fw_stage.h)
+//   k_str       stress half of the operator sweep + CFS-PML (velocity
+//               derivative auxiliaries) + RK update           (fw_stage.h)
+//               (mod_macdrp.F90:790-962, 291-393; mod_abs_npml.F90:1149-2123,
+//               738-1105)
 //   k_src       point-source terms (mod_src.F90:207-414), RK fix-up
 #include "fw_common.h"
 #include "fw_stage.h"
@@ -110,178 +111,6 @@ __global__ void k_surface(SurfArgs g) {
   Tzz[cs] = tz;
   Txz[cs] = tx;
   Tyz[cs] = ty;
-}
-
-// ---------------------------------------------------------------------------
-// CFS-PML correction on one face slab (abs_mod Lx?_Ly?_Lz?, mod_abs_npml.F90
-// :1149-2123) applied as an RK fix-up of the stage outputs written by k_stage
-// (RK is linear in h), plus the auxiliary-variable RK update
-// (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105).
-// Auxiliary variable order per face: x: Vx Vy Vz Txx Txy Txz;
-// y: Vx Vy Vz Tyy Txy Tyz; z: Vx Vy Vz Tzz Txz Tyz.
-// ---------------------------------------------------------------------------
-struct PmlArgs {
-  const float* __restrict__ win;
-  float* __restrict__ wout;
-  float* __restrict__ t;
-  const float* __restrict__ lam;
-  const float* __restrict__ mu;
-  const float* __restrict__ rrho;
-  const float* __restrict__ xi_x;
-  const float* __restrict__ eta_y;
-  const float* __restrict__ zeta_z;
-  const float* __restrict__ rz;
-  const float* __restrict__ rxsin;
-  const float* __restrict__ xcot;
-  const float* __restrict__ pb1;  // 1/B along the face axis (ghosted index)
-  const float* __restrict__ pad;  // A + D/B
-  const float* __restrict__ pdd;  // D/B
-  const float* __restrict__ pin;
-  float* __restrict__ pout;
-  const float* __restrict__ pm;
-  float* __restrict__ pt;
-  long fs, sy, sz, pcnt;
-  int lo0, lo1, lo2, ex0, ex1, ex2;
-  int nk2, freenode;
-  float a, b;
-};
-
-template <int XF, int YF, int ZF, int KIND, int AX>
-__global__ __launch_bounds__(256) void k_pml(PmlArgs g) {
-  const long li = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (li >= g.pcnt) return;
-  const int ii = (int)(li % g.ex0);
-  const long r = li / g.ex0;
-  const int jj = (int)(r % g.ex1);
-  const int kk = (int)(r / g.ex1);
-  const int i = g.lo0 + ii, j = g.lo1 + jj, k = g.lo2 + kk;
-  const long fs = g.fs, sy = g.sy, sz = g.sz;
-  const long c = i + sy * j + sz * (long)k;
-  const float* W = g.win;
-  const float rr = g.rrho[c];
-  const float tc13 = g.lam[c], tc66 = g.mu[c];
-  const float tc11 = tc13 + 2.0f * tc66, tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11;
-  const float tc44 = tc66, tc55 = tc66;
-  const float rzk = g.rz[k], rxs = g.rxsin[i], xc = g.xcot[i];
-  const int ax_idx = AX == 0 ? i : (AX == 1 ? j : k);
-  const float b1 = g.pb1[ax_idx], b1_1 = b1 - 1.0f, ad = g.pad[ax_idx], Dd = g.pdd[ax_idx];
-  float psi[6], dpsi[6];
-#pragma unroll
-  for (int v = 0; v < 6; ++v) psi[v] = g.pin[v * g.pcnt + li];
-  float dh[9];
-#pragma unroll
-  for (int f = 0; f < 9; ++f) dh[f] = 0.f;
-  const bool surf = g.freenode && k == g.nk2;
-
-  if constexpr (AX == 0) {
-    const float xi = g.xi_x[i];
-    float D;
-    D = (dop<XF>(W + TXX * fs + c, 1) * xi + 2.0f * W[TXZ * fs + c]) * rzk;
-    dh[VX] = (b1_1 * D - b1 * psi[3]) * rr;
-    dpsi[3] = Dd * D - ad * psi[3];
-    D = (dop<XF>(W + TXY * fs + c, 1) * xi + W[TYZ * fs + c]) * rzk;
-    dh[VY] = (b1_1 * D - b1 * psi[4]) * rr;
-    dpsi[4] = Dd * D - ad * psi[4];
-    D = (dop<XF>(W + TXZ * fs + c, 1) * xi - W[TXX * fs + c] + W[TZZ * fs + c]) * rzk;
-    dh[VZ] = (b1_1 * D - b1 * psi[5]) * rr;
-    dpsi[5] = Dd * D - ad * psi[5];
-    const float DxVx = (dop<XF>(W + VX * fs + c, 1) * xi + W[VZ * fs + c]) * rzk;
-    dh[TXX] = b1_1 * tc11 * DxVx - b1 * tc11 * psi[0];
-    dh[TYY] = b1_1 * tc12 * DxVx - b1 * tc12 * psi[0];
-    dh[TZZ] = b1_1 * tc13 * DxVx - b1 * tc13 * psi[0];
-    dpsi[0] = Dd * DxVx - ad * psi[0];
-    D = dop<XF>(W + VY * fs + c, 1) * xi * rzk;
-    dh[TXY] = b1_1 * tc66 * D - b1 * tc66 * psi[1];
-    dpsi[1] = Dd * D - ad * psi[1];
-    D = (dop<XF>(W + VZ * fs + c, 1) * xi - W[VX * fs + c]) * rzk;
-    dh[TXZ] = b1_1 * tc55 * D - b1 * tc55 * psi[2];
-    dpsi[2] = Dd * D - ad * psi[2];
-    if (surf) {  // mod_abs_npml.F90:1303-1333 (isotropic branch)
-      dh[TXX] += -b1_1 * tc13 * tc13 / tc33 * DxVx + b1 * tc13 * tc13 / tc33 * psi[0];
-      dh[TYY] += -b1_1 * tc23 * tc13 / tc33 * DxVx + b1 * tc23 * tc13 / tc33 * psi[0];
-    }
-  } else if constexpr (AX == 1) {
-    const float et = g.eta_y[j];
-    float D;
-    D = (dop<YF>(W + TXY * fs + c, sy) * et * rxs + (W[TXX * fs + c] - W[TYY * fs + c]) * xc + W[TXZ * fs + c]) * rzk;
-    dh[VX] = (b1_1 * D - b1 * psi[4]) * rr;
-    dpsi[4] = Dd * D - ad * psi[4];
-    D = (dop<YF>(W + TYY * fs + c, sy) * et * rxs + 2.0f * W[TXY * fs + c] * xc + 2.0f * W[TYZ * fs + c]) * rzk;
-    dh[VY] = (b1_1 * D - b1 * psi[3]) * rr;
-    dpsi[3] = Dd * D - ad * psi[3];
-    D = (dop<YF>(W + TYZ * fs + c, sy) * et * rxs + W[TXZ * fs + c] * xc + (W[TZZ * fs + c] - W[TYY * fs + c])) * rzk;
-    dh[VZ] = (b1_1 * D - b1 * psi[5]) * rr;
-    dpsi[5] = Dd * D - ad * psi[5];
-    D = (dop<YF>(W + VX * fs + c, sy) * et * rxs - W[VY * fs + c] * xc) * rzk;
-    dh[TXY] = b1_1 * tc66 * D - b1 * tc66 * psi[0];
-    dpsi[0] = Dd * D - ad * psi[0];
-    const float DyVy = (dop<YF>(W + VY * fs + c, sy) * et * rxs + W[VX * fs + c] * xc + W[VZ * fs + c]) * rzk;
-    dh[TXX] = b1_1 * tc12 * DyVy - b1 * tc12 * psi[1];
-    dh[TYY] = b1_1 * tc22 * DyVy - b1 * tc22 * psi[1];
-    dh[TZZ] = b1_1 * tc23 * DyVy - b1 * tc23 * psi[1];
-    dpsi[1] = Dd * DyVy - ad * psi[1];
-    D = (dop<YF>(W + VZ * fs + c, sy) * et * rxs - W[VY * fs + c]) * rzk;
-    dh[TYZ] = b1_1 * tc44 * D - b1 * tc44 * psi[2];
-    dpsi[2] = Dd * D - ad * psi[2];
-    if (surf) {
-      dh[TXX] += -b1_1 * tc13 * tc23 / tc33 * DyVy + b1 * tc13 * tc23 / tc33 * psi[1];
-      dh[TYY] += -b1_1 * tc23 * tc23 / tc33 * DyVy + b1 * tc23 * tc23 / tc33 * psi[1];
-    }
-  } else {
-    const float ze = g.zeta_z[k];
-    float D;
-    D = dop<ZF>(W + TXZ * fs + c, sz) * ze;
-    dh[VX] = (b1_1 * D - b1 * psi[4]) * rr;
-    dpsi[4] = Dd * D - ad * psi[4];
-    D = dop<ZF>(W + TYZ * fs + c, sz) * ze;
-    dh[VY] = (b1_1 * D - b1 * psi[5]) * rr;
-    dpsi[5] = Dd * D - ad * psi[5];
-    D = dop<ZF>(W + TZZ * fs + c, sz) * ze;
-    dh[VZ] = (b1_1 * D - b1 * psi[3]) * rr;
-    dpsi[3] = Dd * D - ad * psi[3];
-    D = dop<ZF>(W + VX * fs + c, sz) * ze;
-    dh[TXZ] = b1_1 * tc55 * D - b1 * tc55 * psi[0];
-    dpsi[0] = Dd * D - ad * psi[0];
-    D = dop<ZF>(W + VY * fs + c, sz) * ze;
-    dh[TYZ] = b1_1 * tc44 * D - b1 * tc44 * psi[1];
-    dpsi[1] = Dd * D - ad * psi[1];
-    D = dop<ZF>(W + VZ * fs + c, sz) * ze;
-    dh[TXX] = b1_1 * tc13 * D - b1 * tc13 * psi[2];
-    dh[TYY] = b1_1 * tc23 * D - b1 * tc23 * psi[2];
-    dh[TZZ] = b1_1 * tc33 * D - b1 * tc33 * psi[2];
-    dpsi[2] = Dd * D - ad * psi[2];
-  }
-  // fix-up of the stage outputs
-  constexpr unsigned MASK = AX == 0   ? ((1u << VX) | (1u << VY) | (1u << VZ) | (1u << TXX) | (1u << TYY) |
-                                       (1u << TZZ) | (1u << TXY) | (1u << TXZ))
-                            : AX == 1 ? ((1u << VX) | (1u << VY) | (1u << VZ) | (1u << TXX) | (1u << TYY) |
-                                       (1u << TZZ) | (1u << TXY) | (1u << TYZ))
-                                      : ((1u << VX) | (1u << VY) | (1u << VZ) | (1u << TXX) | (1u << TYY) |
-                                       (1u << TZZ) | (1u << TXZ) | (1u << TYZ));
-#pragma unroll
-  for (int f = 0; f < 9; ++f) {
-    if (!(MASK & (1u << f))) continue;
-    if constexpr (KIND == 2) {
-      g.wout[f * fs + c] += g.b * dh[f];
-    } else {
-      g.wout[f * fs + c] += g.a * dh[f];
-      g.t[f * fs + c] += g.b * dh[f];
-    }
-  }
-  // auxiliary RK
-#pragma unroll
-  for (int v = 0; v < 6; ++v) {
-    const long q = v * g.pcnt + li;
-    if constexpr (KIND == 0) {
-      g.pout[q] = psi[v] + g.a * dpsi[v];
-      g.pt[q] = psi[v] + g.b * dpsi[v];
-    } else if constexpr (KIND == 1) {
-      g.pout[q] = g.pm[q] + g.a * dpsi[v];
-      g.pt[q] = g.pt[q] + g.b * dpsi[v];
-    } else {
-      g.pout[q] = g.pt[q] + g.b * dpsi[v];
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -473,6 +302,10 @@ struct fw_solver_s {
   float* d_msurf = nullptr;
   int* d_fluidcol = nullptr;  // per (i, j) column: 1 if any cell has mu <= SEIS_ZERO
   int tilecfg = 0;            // stage-kernel tile shape index
+  int4* d_items = nullptr;    // stage work items: interior items, then PML-shell items
+  int n_lean = 0, n_pml = 0;
+  hipStream_t stream2 = nullptr;  // shell items run here, concurrently with the interior
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // host copies
   std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
   std::vector<float> hlam, hmu, hrho;
@@ -550,6 +383,12 @@ int fw_create(const fw_config* cfg, fw_handle* out) {
   if (cfg->abs_number[5] > 0) h->freenode = 0;  // mod_abs_npml.F90:182-184
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete h; return fail(nullptr, FW_ERR_HIP, "hipStreamCreate failed"); }
+  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
+    fw_destroy(h);
+    return fail(nullptr, FW_ERR_HIP, "hipStreamCreate/hipEventCreate failed");
+  }
   const size_t setbytes = sizeof(float) * (size_t)(9 * h->fs);
   e = hipMalloc(&h->d_sets, 4 * setbytes + 64 * sizeof(float));
   if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc wavefield sets failed"); }
@@ -578,6 +417,11 @@ int fw_destroy(fw_handle h) {
   hipFree(h->d_pt); hipFree(h->d_rec);
   for (auto& s : h->snaps) hipFree(s.d);
   for (auto& e : h->ev) hipEventDestroy(e);
+  if (h->stream2) hipStreamSynchronize(h->stream2);
+  hipFree(h->d_items);
+  if (h->ev_fork) hipEventDestroy(h->ev_fork);
+  if (h->ev_join) hipEventDestroy(h->ev_join);
+  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return FW_OK;
@@ -980,6 +824,8 @@ static int zero_state(fw_handle h) {
   return FW_OK;
 }
 
+static int build_items(fw_handle h);
+
 int fw_init(fw_handle h) {
   if (!h) return FW_ERR_ARG;
   if (!h->have_grid || !h->have_media) return fail(h, FW_ERR_STATE, "fw_init: grid and media must be set first");
@@ -1014,7 +860,9 @@ int fw_init(fw_handle h) {
     for (int d = 0; d < 3; ++d) { f.ex[d] = f.hi[d] - f.lo[d] + 1; f.cnt *= f.ex[d]; }
     HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt));
   }
-  int rc = build_sources(h);
+  int rc = build_items(h);
+  if (rc) return rc;
+  rc = build_sources(h);
   if (rc) return rc;
   rc = zero_state(h);
   if (rc) return rc;
@@ -1040,38 +888,79 @@ int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
 // Kernel dispatch tables over the 8 F/B patterns and 3 RK kinds.
 // ---------------------------------------------------------------------------
 typedef void (*StageFn)(StageArgs);
-typedef void (*PmlFn)(PmlArgs);
 
-template <int P, int K, int C>
+template <int P, int K, int C, bool M>
 static StageFn vel_fn() {
-  return k_vel<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C]>;
+  return k_vel<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C], M>;
 }
-template <int P, int K, int C>
+template <int P, int K, int C, bool M>
 static StageFn str_fn() {
-  return k_str<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C]>;
+  return k_str<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C], M>;
 }
-template <int P, int K, int AX>
-static PmlFn pml_fn() {
-  return k_pml<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, AX>;
+#define VROW(P, C, M) {vel_fn<P, 0, C, M>(), vel_fn<P, 1, C, M>(), vel_fn<P, 2, C, M>()}
+#define SROW(P, C, M) {str_fn<P, 0, C, M>(), str_fn<P, 1, C, M>(), str_fn<P, 2, C, M>()}
+#define VTAB(C, M) {VROW(0, C, M), VROW(1, C, M), VROW(2, C, M), VROW(3, C, M), \
+                    VROW(4, C, M), VROW(5, C, M), VROW(6, C, M), VROW(7, C, M)}
+#define STAB(C, M) {SROW(0, C, M), SROW(1, C, M), SROW(2, C, M), SROW(3, C, M), \
+                    SROW(4, C, M), SROW(5, C, M), SROW(6, C, M), SROW(7, C, M)}
+static StageFn vel_table(int c, int pml, int p, int kind) {
+  static const StageFn T[NTILECFG][2][8][3] = {{VTAB(0, false), VTAB(0, true)}, {VTAB(1, false), VTAB(1, true)}};
+  return T[c][pml][p][kind];
 }
-#define VROW(P, C) {vel_fn<P, 0, C>(), vel_fn<P, 1, C>(), vel_fn<P, 2, C>()}
-#define SROW(P, C) {str_fn<P, 0, C>(), str_fn<P, 1, C>(), str_fn<P, 2, C>()}
-#define VTAB(C) {VROW(0, C), VROW(1, C), VROW(2, C), VROW(3, C), VROW(4, C), VROW(5, C), VROW(6, C), VROW(7, C)}
-#define STAB(C) {SROW(0, C), SROW(1, C), SROW(2, C), SROW(3, C), SROW(4, C), SROW(5, C), SROW(6, C), SROW(7, C)}
-static StageFn vel_table(int c, int p, int kind) {
-  static const StageFn T[NTILECFG][8][3] = {VTAB(0), VTAB(1)};
-  return T[c][p][kind];
+static StageFn str_table(int c, int pml, int p, int kind) {
+  static const StageFn T[NTILECFG][2][8][3] = {{STAB(0, false), STAB(0, true)}, {STAB(1, false), STAB(1, true)}};
+  return T[c][pml][p][kind];
 }
-static StageFn str_table(int c, int p, int kind) {
-  static const StageFn T[NTILECFG][8][3] = {STAB(0), STAB(1)};
-  return T[c][p][kind];
-}
-#define PROW(P, K) {pml_fn<P, K, 0>(), pml_fn<P, K, 1>(), pml_fn<P, K, 2>()}
-#define PROW3(P) {PROW(P, 0), PROW(P, 1), PROW(P, 2)}
-static PmlFn pml_table(int p, int kind, int ax) {
-  static const PmlFn T[8][3][3] = {PROW3(0), PROW3(1), PROW3(2), PROW3(3),
-                                   PROW3(4), PROW3(5), PROW3(6), PROW3(7)};
-  return T[p][kind][ax];
+
+// Work items of the stage kernels: TX x TY tiles over chunks of k planes,
+// split into items that touch a CFS-PML slab (run by the PML-specialised
+// kernels) and interior items (lean kernels).  Each list is ordered by
+// chunk, tile row, tile column.
+static int build_items(fw_handle h) {
+  const int TY = BYT * TILE_RY[h->tilecfg];
+  const int ntx = (h->cfg.ni + TX - 1) / TX, nty = (h->cfg.nj + TY - 1) / TY;
+  const PmlFace* F = h->face;
+  const int zlo = F[4].on ? F[4].hi[2] : h->nk1 - 1;  // last plane of the z1 slab
+  const int zhi = F[5].on ? F[5].lo[2] : h->nk2 + 1;  // first plane of the z2 slab
+  const int mid = std::max(0, zhi - zlo - 1);
+  const int ntile = ntx * nty;
+  int nch = std::max(1, std::min(std::max(1, mid / 16), (2048 + ntile - 1) / ntile));
+  const int cs = std::max(1, (std::max(mid, 1) + nch - 1) / nch);
+  std::vector<int4> lean, pml;
+  auto add_range = [&](std::vector<int4>& v, int i0, int j0, int ka, int kb) {
+    for (int k0 = ka; k0 <= kb; k0 += cs) v.push_back(make_int4(i0, j0, k0, std::min(k0 + cs - 1, kb)));
+  };
+  for (int by = 0; by < nty; ++by)
+    for (int bx = 0; bx < ntx; ++bx) {
+      const int i0 = h->ni1 + bx * TX, j0 = h->nj1 + by * TY;
+      const bool shell = (F[0].on && i0 <= F[0].hi[0]) || (F[1].on && i0 + TX - 1 >= F[1].lo[0]) ||
+                         (F[2].on && j0 <= F[2].hi[1]) || (F[3].on && j0 + TY - 1 >= F[3].lo[1]);
+      if (shell) {
+        add_range(pml, i0, j0, h->nk1, h->nk2);
+      } else {
+        if (F[4].on) add_range(pml, i0, j0, h->nk1, zlo);
+        if (mid > 0) add_range(lean, i0, j0, zlo + 1, zhi - 1);
+        if (F[5].on) add_range(pml, i0, j0, zhi, h->nk2);
+      }
+    }
+  auto order = [](const int4& a, const int4& b) {
+    if (a.z != b.z) return a.z < b.z;
+    if (a.y != b.y) return a.y < b.y;
+    return a.x < b.x;
+  };
+  std::stable_sort(lean.begin(), lean.end(), order);
+  std::stable_sort(pml.begin(), pml.end(), order);
+  hipFree(h->d_items);
+  h->d_items = nullptr;
+  h->n_lean = (int)lean.size();
+  h->n_pml = (int)pml.size();
+  std::vector<int4> all(lean);
+  all.insert(all.end(), pml.begin(), pml.end());
+  if (!all.empty()) {
+    HIPCHK(hipMalloc(&h->d_items, sizeof(int4) * all.size()));
+    HIPCHK(hipMemcpy(h->d_items, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
+  }
+  return FW_OK;
 }
 
 extern "C" {
@@ -1113,39 +1002,43 @@ static int launch_stage(fw_handle h, int ntime, int st) {
   g.ni1 = h->ni1; g.ni2 = h->ni2; g.nj1 = h->nj1; g.nj2 = h->nj2; g.nk1 = h->nk1; g.nk2 = h->nk2;
   g.freenode = h->freenode; g.a = a; g.b = b;
   g.nx = h->nx; g.ny = h->ny; g.fluidcol = h->d_fluidcol;
-  g.ntx = (h->cfg.ni + TX - 1) / TX;
-  const int tc = h->tilecfg, tyr = BYT * TILE_RY[tc];
-  g.nty = (h->cfg.nj + tyr - 1) / tyr;
-  const int ntile = g.ntx * g.nty;
-  int nchunk = std::max(1, std::min(h->cfg.nk / 16, (1024 + ntile - 1) / ntile));
-  g.kchunk = (h->cfg.nk + nchunk - 1) / nchunk;
-  g.nch = (h->cfg.nk + g.kchunk - 1) / g.kchunk;
-  const dim3 sgrid((unsigned)(ntile * g.nch)), sblk(TX, BYT);
+  g.n1 = h->nx + h->ny + h->nz;
+  g.pb1 = h->d_pml; g.pad = h->d_pml + g.n1; g.pdd = h->d_pml + 2 * g.n1;
+  g.setin = IN[st]; g.setout = OUT[st];
+  g.pml = 0;
+  for (int n = 0; n < 6; ++n) {
+    const PmlFace& f = h->face[n];
+    FaceArgs& fa = g.face[n];
+    fa.on = f.on ? 1 : 0;
+    g.pml |= fa.on;
+    fa.d = f.d; fa.cnt = f.cnt;
+    fa.lo0 = f.lo[0]; fa.lo1 = f.lo[1]; fa.lo2 = f.lo[2];
+    fa.hi0 = f.hi[0]; fa.hi1 = f.hi[1]; fa.hi2 = f.hi[2];
+    fa.ex0 = f.ex[0]; fa.ex1 = f.ex[1];
+  }
+  const int tc = h->tilecfg;
+  const dim3 sblk(TX, BYT);
   if (h->timing) hipEventRecord(h->ev[2 * h->nstage], h->stream);
-  hipLaunchKernelGGL(vel_table(tc, pat, kind), sgrid, sblk, 0, h->stream, g);
-  hipLaunchKernelGGL(str_table(tc, pat, kind), sgrid, sblk, 0, h->stream, g);
+  if (h->n_pml > 0) {  // shell items on the second stream, concurrent with the interior
+    hipEventRecord(h->ev_fork, h->stream);
+    hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
+    StageArgs gp = g;
+    gp.items = h->d_items + h->n_lean;
+    gp.nitems = h->n_pml;
+    hipLaunchKernelGGL(vel_table(tc, 1, pat, kind), dim3(h->n_pml), sblk, 0, h->stream2, gp);
+    hipLaunchKernelGGL(str_table(tc, 1, pat, kind), dim3(h->n_pml), sblk, 0, h->stream2, gp);
+    hipEventRecord(h->ev_join, h->stream2);
+  }
+  if (h->n_lean > 0) {
+    g.items = h->d_items;
+    g.nitems = h->n_lean;
+    hipLaunchKernelGGL(vel_table(tc, 0, pat, kind), dim3(h->n_lean), sblk, 0, h->stream, g);
+    hipLaunchKernelGGL(str_table(tc, 0, pat, kind), dim3(h->n_lean), sblk, 0, h->stream, g);
+  }
+  if (h->n_pml > 0) hipStreamWaitEvent(h->stream, h->ev_join, 0);
   if (h->timing) {
     hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
     h->nstage++;
-  }
-  // PML faces
-  const long n1 = h->nx + h->ny + h->nz;
-  for (int n = 0; n < 6; ++n) {
-    const PmlFace& f = h->face[n];
-    if (!f.on) continue;
-    const int ax = n / 2;
-    const long axoff = ax == 0 ? 0 : (ax == 1 ? h->nx : h->nx + h->ny);
-    PmlArgs p{};
-    p.win = win; p.wout = wout; p.t = T;
-    p.lam = h->lam; p.mu = h->mu; p.rrho = h->rrho;
-    p.xi_x = h->xi_x; p.eta_y = h->eta_y; p.zeta_z = h->zeta_z; p.rz = h->rz; p.rxsin = h->rxsin; p.xcot = h->xcot;
-    p.pb1 = h->d_pml + axoff; p.pad = h->d_pml + n1 + axoff; p.pdd = h->d_pml + 2 * n1 + axoff;
-    const long set = 6 * f.cnt;
-    p.pin = f.d + IN[st] * set; p.pout = f.d + OUT[st] * set; p.pm = f.d; p.pt = f.d + 3 * set;
-    p.fs = h->fs; p.sy = h->sy; p.sz = h->sz; p.pcnt = f.cnt;
-    p.lo0 = f.lo[0]; p.lo1 = f.lo[1]; p.lo2 = f.lo[2]; p.ex0 = f.ex[0]; p.ex1 = f.ex[1]; p.ex2 = f.ex[2];
-    p.nk2 = h->nk2; p.freenode = h->freenode; p.a = a; p.b = b;
-    hipLaunchKernelGGL(pml_table(pat, kind, ax), dim3((unsigned)((f.cnt + 255) / 256)), dim3(256), 0, h->stream, p);
   }
   // point sources
   if (h->n_srccell > 0) {

@@ -24,6 +24,16 @@ namespace fw {
 
 constexpr int TX = 64;
 
+// One CFS-PML face slab (x1 x2 y1 y2 z1 z2): inclusive bounds, extents and
+// the auxiliary variables, 4 register sets (A B C T) x 6 variables x cnt.
+// Variable order: x: Vx Vy Vz Txx Txy Txz; y: Vx Vy Vz Tyy Txy Tyz;
+// z: Vx Vy Vz Tzz Txz Tyz (the psi of the derivative of that field).
+struct FaceArgs {
+  float* d;
+  long cnt;
+  int lo0, lo1, lo2, hi0, hi1, hi2, ex0, ex1, on;
+};
+
 struct StageArgs {
   const float* __restrict__ win;
   float* __restrict__ wout;
@@ -43,22 +53,155 @@ struct StageArgs {
   long fs, sy, sz, ps;
   int nx, ny;
   int ni1, ni2, nj1, nj2, nk1, nk2;
-  int kchunk, freenode;
-  int ntx, nty, nch;
+  int freenode;
+  const int4* __restrict__ items;  // work items: (i0, j0, k0, k1) of a tile over a chunk of planes
+  int nitems;
   float a, b;
+  // CFS-PML: per-axis profiles 1/B, A + D/B, D/B (x: nx, y: ny, z: nz entries)
+  const float* __restrict__ pb1;
+  const float* __restrict__ pad;
+  const float* __restrict__ pdd;
+  int n1, setin, setout, pml;
+  FaceArgs face[6];
 };
 
+__device__ __forceinline__ long face_index(const FaceArgs& f, int i, int j, int k) {
+  return (long)(i - f.lo0) + (long)f.ex0 * ((j - f.lo1) + (long)f.ex1 * (k - f.lo2));
+}
+__device__ __forceinline__ bool in_face(const FaceArgs& f, int i, int j, int k) {
+  return f.on && i >= f.lo0 && i <= f.hi0 && j >= f.lo1 && j <= f.hi1 && k >= f.lo2 && k <= f.hi2;
+}
+// auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105)
+template <int KIND>
+__device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, long li, int v, float psi, float dpsi) {
+  const long S = 6 * f.cnt, q = v * f.cnt + li;
+  if constexpr (KIND == 0) {
+    f.d[g.setout * S + q] = psi + g.a * dpsi;
+    f.d[3 * S + q] = psi + g.b * dpsi;
+  } else if constexpr (KIND == 1) {
+    f.d[g.setout * S + q] = f.d[q] + g.a * dpsi;
+    f.d[3 * S + q] = f.d[3 * S + q] + g.b * dpsi;
+  } else {
+    f.d[g.setout * S + q] = f.d[3 * S + q] + g.b * dpsi;
+  }
+}
+__device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, long li, int v) {
+  return f.d[g.setin * 6 * f.cnt + v * f.cnt + li];
+}
+
+// CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
+// reference's face order x1 x2 y1 y2 z1 z2 (mod_abs_npml.F90:1149-2123, the
+// stress-derivative half).  D* are the raw one-sided derivatives times the
+// 1-D metric (before fluid zeroing).
+template <int KIND>
+__device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k, float h[3], float rr, float DxTxx,
+                                        float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz, float DzTzz,
+                                        float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz, float Txz,
+                                        float Tyz, float rzk, float rxs, float xc) {
+  for (int n = 0; n < 6; ++n) {
+    const FaceArgs& f = g.face[n];
+    if (!in_face(f, i, j, k)) continue;
+    const int ax = n >> 1;
+    const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
+    const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
+    const long li = face_index(f, i, j, k);
+    const float p3 = aux_in(f, g, li, 3), p4 = aux_in(f, g, li, 4), p5 = aux_in(f, g, li, 5);
+    float D0, D1, D2;  // corrected derivatives feeding hVx, hVy, hVz
+    float q0, q1, q2;  // their psi
+    int v0, v1, v2;
+    if (ax == 0) {
+      D0 = (DxTxx + 2.0f * Txz) * rzk; q0 = p3; v0 = 3;
+      D1 = (DxTxy + Tyz) * rzk; q1 = p4; v1 = 4;
+      D2 = (DxTxz - Txx + Tzz) * rzk; q2 = p5; v2 = 5;
+    } else if (ax == 1) {
+      D0 = (DyTxy * rxs + (Txx - Tyy) * xc + Txz) * rzk; q0 = p4; v0 = 4;
+      D1 = (DyTyy * rxs + 2.0f * Txy * xc + 2.0f * Tyz) * rzk; q1 = p3; v1 = 3;
+      D2 = (DyTyz * rxs + Txz * xc + (Tzz - Tyy)) * rzk; q2 = p5; v2 = 5;
+    } else {
+      D0 = DzTxz; q0 = p4; v0 = 4;
+      D1 = DzTyz; q1 = p5; v1 = 5;
+      D2 = DzTzz; q2 = p3; v2 = 3;
+    }
+    h[0] += (b1_1 * D0 - b1 * q0) * rr;
+    h[1] += (b1_1 * D1 - b1 * q1) * rr;
+    h[2] += (b1_1 * D2 - b1 * q2) * rr;
+    aux_rk<KIND>(f, g, li, v0, q0, Dd * D0 - ad * q0);
+    aux_rk<KIND>(f, g, li, v1, q1, Dd * D1 - ad * q1);
+    aux_rk<KIND>(f, g, li, v2, q2, Dd * D2 - ad * q2);
+  }
+}
+
+// CFS-PML corrections of hT (velocity-derivative half), including the
+// free-surface Hooke fix-ups of the x and y faces (mod_abs_npml.F90:1303-1333).
+// h: Txx Tyy Tzz Txy Txz Tyz.  DzV* are the standard 5-tap z derivatives.
+template <int KIND>
+__device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k, float h[6], float lam, float miu,
+                                        float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
+                                        float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
+                                        float rxs, float xc) {
+  const float tc13 = lam, tc66 = miu, tc11 = tc13 + 2.0f * tc66;
+  const float tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11, tc44 = tc66, tc55 = tc66;
+  const bool surf = g.freenode && k == g.nk2;
+  for (int n = 0; n < 6; ++n) {
+    const FaceArgs& f = g.face[n];
+    if (!in_face(f, i, j, k)) continue;
+    const int ax = n >> 1;
+    const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
+    const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
+    const long li = face_index(f, i, j, k);
+    const float p0 = aux_in(f, g, li, 0), p1 = aux_in(f, g, li, 1), p2 = aux_in(f, g, li, 2);
+    if (ax == 0) {
+      const float D0 = (DxVx + Vz) * rzk, D1 = DxVy * rzk, D2 = (DxVz - Vx) * rzk;
+      h[0] += b1_1 * tc11 * D0 - b1 * tc11 * p0;
+      h[1] += b1_1 * tc12 * D0 - b1 * tc12 * p0;
+      h[2] += b1_1 * tc13 * D0 - b1 * tc13 * p0;
+      h[3] += b1_1 * tc66 * D1 - b1 * tc66 * p1;
+      h[4] += b1_1 * tc55 * D2 - b1 * tc55 * p2;
+      if (surf) {
+        h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
+        h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
+      }
+      aux_rk<KIND>(f, g, li, 0, p0, Dd * D0 - ad * p0);
+      aux_rk<KIND>(f, g, li, 1, p1, Dd * D1 - ad * p1);
+      aux_rk<KIND>(f, g, li, 2, p2, Dd * D2 - ad * p2);
+    } else if (ax == 1) {
+      const float D0 = (DyVx * rxs - Vy * xc) * rzk;
+      const float D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
+      const float D2 = (DyVz * rxs - Vy) * rzk;
+      h[3] += b1_1 * tc66 * D0 - b1 * tc66 * p0;
+      h[0] += b1_1 * tc12 * D1 - b1 * tc12 * p1;
+      h[1] += b1_1 * tc22 * D1 - b1 * tc22 * p1;
+      h[2] += b1_1 * tc23 * D1 - b1 * tc23 * p1;
+      h[5] += b1_1 * tc44 * D2 - b1 * tc44 * p2;
+      if (surf) {
+        h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
+        h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
+      }
+      aux_rk<KIND>(f, g, li, 0, p0, Dd * D0 - ad * p0);
+      aux_rk<KIND>(f, g, li, 1, p1, Dd * D1 - ad * p1);
+      aux_rk<KIND>(f, g, li, 2, p2, Dd * D2 - ad * p2);
+    } else {
+      h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
+      h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
+      h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
+      h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
+      h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
+      aux_rk<KIND>(f, g, li, 0, p0, Dd * DzVx - ad * p0);
+      aux_rk<KIND>(f, g, li, 1, p1, Dd * DzVy - ad * p1);
+      aux_rk<KIND>(f, g, li, 2, p2, Dd * DzVz - ad * p2);
+    }
+  }
+}
+
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD
-// receives a contiguous run of logical tiles (shared halos stay in one L2).
-__device__ __forceinline__ void tile_of_block(const StageArgs& g, int& bx, int& by, int& bz) {
-  const int nb = g.ntx * g.nty * g.nch;
+// receives a contiguous run of work items (items are ordered by chunk, then
+// tile row, then tile column, so shared halos stay in one L2).
+__device__ __forceinline__ int4 item_of_block(const StageArgs& g) {
+  const int nb = g.nitems;
   const int b = blockIdx.x;
   const int q = nb / 8;
   const int L = (b < 8 * q) ? (b % 8) * q + b / 8 : b;
-  bx = L % g.ntx;
-  const int r = L / g.ntx;
-  by = r % g.nty;
-  bz = r / g.nty;
+  return g.items[L];
 }
 
 // one-sided 5-tap operator on LDS: p points at the centre, s is the stride
@@ -145,15 +288,15 @@ constexpr int NTH = 256, BYT = 4;
 // (:917-926), RK update of Vx Vy Vz.
 // LDS: Txx (x), Txy (x, y), Txz (x), Tyy (y), Tyz (y); z window: Tzz Txz Tyz.
 // ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int KIND, int RY>
+template <int XF, int YF, int ZF, int KIND, int RY, bool PML>
 __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
   constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, NL = 5;
   __shared__ float sh[2][NL * PL];  // Txx Txy Txz Tyy Tyz, double buffered
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-  int bx, by, bz;
-  tile_of_block(g, bx, by, bz);
-  const int i0 = g.ni1 + bx * TX, j0 = g.nj1 + by * TY;
-  const int k0 = g.nk1 + bz * g.kchunk, k1 = min(k0 + g.kchunk - 1, g.nk2);
+  const int4 it = item_of_block(g);
+  const int i0 = it.x, j0 = it.y, k0 = it.z, k1 = it.w;
+  const bool tile_pml = PML && (g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && i0 + TX - 1 >= g.face[1].lo0) ||
+                        (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
   const float* __restrict__ W = g.win;
@@ -261,6 +404,7 @@ __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
     }
     __syncthreads();
     const float ze = g.zeta_z[k], rzk = g.rz[k];
+    const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       if (comp[q]) {
@@ -275,6 +419,7 @@ __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
         const float DzTzz = dz5<ZF>(w[q][0]) * ze;
         float DzTxz = dz5<ZF>(w[q][1]) * ze;
         float DzTyz = dz5<ZF>(w[q][2]) * ze;
+        const float rDxTxy = DxTxy, rDxTxz = DxTxz, rDyTxy = DyTxy, rDyTyz = DyTyz, rDzTxz = DzTxz, rDzTyz = DzTyz;
         if (fl[q]) {
           DxTxy = 0.f; DxTxz = 0.f; DyTxy = 0.f; DyTyz = 0.f; DzTxz = 0.f; DzTyz = 0.f;
         }
@@ -284,6 +429,9 @@ __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
         h[0] = rr[q] * (DxTxx * rzk + DyTxy * rzk * rxs + DzTxz + (3.0f * Txz + Txx * xc - Tyy * xc) * rzk);
         h[1] = rr[q] * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
         h[2] = rr[q] * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
+        if constexpr (PML)
+          if (pml_here) pml_vel<KIND>(g, i, j0 + ty + BYT * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
+                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
@@ -319,15 +467,15 @@ __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
 // Tzz Txz Tyz are the pre-image stresses saved by k_surface.
 // LDS: Vx Vy Vz (x and y halos); z window: Vx Vy Vz.
 // ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int KIND, int RY>
+template <int XF, int YF, int ZF, int KIND, int RY, bool PML>
 __global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
   constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH;
   __shared__ float sh[2][3 * PL];  // Vx Vy Vz, double buffered
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-  int bx, by, bz;
-  tile_of_block(g, bx, by, bz);
-  const int i0 = g.ni1 + bx * TX, j0 = g.nj1 + by * TY;
-  const int k0 = g.nk1 + bz * g.kchunk, k1 = min(k0 + g.kchunk - 1, g.nk2);
+  const int4 it = item_of_block(g);
+  const int i0 = it.x, j0 = it.y, k0 = it.z, k1 = it.w;
+  const bool tile_pml = PML && (g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && i0 + TX - 1 >= g.face[1].lo0) ||
+                        (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
   const float* __restrict__ W = g.win;
@@ -419,6 +567,7 @@ __global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
     }
     __syncthreads();
     const float ze = g.zeta_z[k], rzk = g.rz[k];
+    const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       if (comp[q]) {
@@ -430,13 +579,12 @@ __global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
         const float DyVx = dlds<YF>(s + o, LW) * et[q];
         const float DyVy = dlds<YF>(s + PL + o, LW) * et[q];
         const float DyVz = dlds<YF>(s + 2 * PL + o, LW) * et[q];
-        float DzVx, DzVy, DzVz;
+        const float sDzVx = dz5<ZF>(w[q][0]) * ze, sDzVy = dz5<ZF>(w[q][1]) * ze, sDzVz = dz5<ZF>(w[q][2]) * ze;
+        float DzVx = sDzVx, DzVy = sDzVy, DzVz = sDzVz;
         if (g.freenode && k == g.nk2 - 1) {
           DzVx = dz22<ZF>(w[q][0]) * ze; DzVy = dz22<ZF>(w[q][1]) * ze; DzVz = dz22<ZF>(w[q][2]) * ze;
         } else if (g.freenode && k == g.nk2 - 2) {
           DzVx = dz24<ZF>(w[q][0]) * ze; DzVy = dz24<ZF>(w[q][1]) * ze; DzVz = dz24<ZF>(w[q][2]) * ze;
-        } else {
-          DzVx = dz5<ZF>(w[q][0]) * ze; DzVy = dz5<ZF>(w[q][1]) * ze; DzVz = dz5<ZF>(w[q][2]) * ze;
         }
         const float Vx = zc<ZF>(w[q][0]), Vy = zc<ZF>(w[q][1]), Vz = zc<ZF>(w[q][2]);
         const float lam = la[q], miu = mu_[q], lam2mu = lam + 2.0f * miu;
@@ -458,6 +606,9 @@ __global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
         h[3] = 2.0f * miu * E12;
         h[4] = 2.0f * miu * E13;
         h[5] = 2.0f * miu * E23;
+        if constexpr (PML)
+          if (pml_here) pml_str<KIND>(g, i, j0 + ty + BYT * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
+                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc);
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
           const long e = (TXX + f) * fs + c;

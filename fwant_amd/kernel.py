@@ -1,0 +1,147 @@
+"""Sensitivity-kernel time integral on the GPU (host mirror of
+code/srcF.kernel/SI_ker_sta.F90's numeric subroutines).
+
+Functions take device arrays (fwant_amd._lib.DeviceArray, allocated through
+the library's own HIP runtime) with the reference's series semantics; the
+layout is [component][sample][point] (point fastest), i.e. the
+transpose-free form of the reference's (i, j, k, n) arrays.  Work runs on
+the default stream; the synchronous copies of DeviceArray order it.
+
+  stress2strain(E, lam, mu)                 SI_ker_sta.F90:1185-1211
+  sgt2scatter(R, S, lam, mu, dt)            :1257-1337 (one receiver component)
+  scatter_filter(w0, b, a) / scatter_filtfilt   :1433-1475
+  scatter2kernel(V, U, Sa, Sb, nt1, nt2, dt, M0)  :1375-1431
+  scatter_rotate(X, Y, Z, rot)              :1354-1373
+  pair_kernel(...)                          the per-block pipeline of :574-798
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import DeviceArray, kcheck
+
+
+def _ptr(t):
+    if not isinstance(t, DeviceArray):
+        raise ValueError("DeviceArray required")
+    return C.c_void_p(t.ptr)
+
+
+def _series(t):
+    """(ncomp, knt, npt) tensor -> base pointer, component stride, time stride."""
+    nc, knt, npt = t.shape
+    return _ptr(t), knt * npt, npt, knt, npt
+
+
+def _stream():
+    return None
+
+
+def stress2strain(E, lam, mu):
+    """In-place stress -> strain of a (6, knt, npt) float32 series."""
+    L = _lib.lib()
+    p, cs, ts, knt, npt = _series(E)
+    kcheck(L.fwk_stress2strain(p, cs, ts, npt, knt, _ptr(lam), _ptr(mu), _stream()), "fwk_stress2strain")
+    return E
+
+
+def sgt2scatter(R, S, lam, mu, dt):
+    """alpha and beta scattered series (knt, npt) of one receiver component."""
+    L = _lib.lib()
+    rp, rcs, rts, knt, npt = _series(R)
+    sp, scs, sts, knt2, npt2 = _series(S)
+    if (knt, npt) != (knt2, npt2):
+        raise ValueError("R and S shapes differ")
+    A = DeviceArray((knt, npt))
+    B = DeviceArray((knt, npt))
+    kcheck(L.fwk_scatter(rp, rcs, rts, sp, scs, sts, npt, knt, float(dt), _ptr(lam), _ptr(mu), _ptr(A), _ptr(B),
+                         _stream()), "fwk_scatter")
+    return A, B
+
+
+def _filter(mode, w0, b, a):
+    L = _lib.lib()
+    knt, npt = w0.shape
+    b = np.ascontiguousarray(b, np.float64)
+    a = np.ascontiguousarray(a, np.float64)
+    if len(a) != len(b):
+        raise ValueError("filter a and b must have the same length (NOD1)")
+    w = DeviceArray(w0.shape)
+    kcheck(L.fwk_filter(mode, npt, knt, len(b), _lib.dptr(b), _lib.dptr(a), _ptr(w0), _ptr(w), _stream()),
+           "fwk_filter")
+    return w
+
+
+def scatter_filter(w0, b, a):
+    """P1: causal IIR with zero front padding and odd end padding."""
+    return _filter(1, w0, b, a)
+
+
+def scatter_filtfilt(w0, b, a):
+    """P2: zero-phase forward-backward IIR with odd padding (NFACT = 3*NOD)."""
+    return _filter(2, w0, b, a)
+
+
+def scatter2kernel(V, U, Sa, Sb, nt1, nt2, dt, M0):
+    """Phase/amplitude kernels (Kap, Kaq, Kbp, Kbq), each (npt,); V, U host series."""
+    L = _lib.lib()
+    knt, npt = Sa.shape
+    V = np.ascontiguousarray(V, np.float32)
+    U = np.ascontiguousarray(U, np.float32)
+    out = [DeviceArray((npt,)) for _ in range(4)]
+    kcheck(L.fwk_scatter2kernel(npt, knt, int(nt1), int(nt2), float(dt), float(M0), _lib.fptr(V), _lib.fptr(U),
+                                _ptr(Sa), _ptr(Sb), *[_ptr(o) for o in out], _stream()), "fwk_scatter2kernel")
+    return out
+
+
+def scatter_rotate(X, Y, Z, rot):
+    """In-place rotation of three series by the 3x3 matrix rot (rot[r, c])."""
+    L = _lib.lib()
+    r = np.ascontiguousarray(np.asarray(rot, np.float32).ravel(order="F"))
+    kcheck(L.fwk_rotate(X.numel(), _ptr(X), _ptr(Y), _ptr(Z), _lib.fptr(r), _stream()), "fwk_rotate")
+
+
+def filter_series_host(x, b, a, mode):
+    """syn_filter (mode 1) / syn_filtfilt (mode 2) of one host series
+    (SI_ker_sta.F90:1477-1557): FP64 IIR with the same padding as the
+    scattered series."""
+    x = np.asarray(x, np.float64)
+    b = np.asarray(b, np.float64)
+    a = np.asarray(a, np.float64)
+    nod1 = len(b)
+    nfact = 3 * (nod1 - 1)
+    knt = len(x)
+    end = 2 * x[-1] - x[knt - 1 - np.arange(1, nfact + 1)]
+    front = np.zeros(nfact) if mode == 1 else (2 * x[0] - x[1:nfact + 1])[::-1]
+    v = np.concatenate([front, x, end])
+
+    def iir(u):
+        y = np.zeros_like(u)
+        for n in range(len(u)):
+            acc = b[0] * u[n]
+            for m in range(1, min(nod1, n + 1)):
+                acc = acc + b[m] * u[n - m] - a[m] * y[n - m]
+            y[n] = acc / a[0]
+        return y
+
+    y = iir(v) if mode == 1 else iir(iir(v)[::-1])[::-1]
+    return y[nfact:nfact + knt].astype(np.float32)
+
+
+def pair_kernel(Rx, Ry, Rz, S, lam, mu, dt, V, U, nt1, nt2, M0, comp, filt=None):
+    """One receiver-component kernel of a station-event pair (SI_ker_sta.F90
+    :574-798): stress SGTs -> strain, scattered series, optional filter
+    (("P1"|"P2", b, a)), trapezoid kernel.  Rx/Ry/Rz/S are (6, knt, npt)
+    stress series (Rx..Rz may be None when unused); comp in {0, 1, 2}
+    selects the receiver component (x, y, z)."""
+    R = (Rx, Ry, Rz)[comp]
+    Rs = stress2strain(R.copy(), lam, mu)
+    Ss = stress2strain(S.copy(), lam, mu)
+    A, B = sgt2scatter(Rs, Ss, lam, mu, dt)
+    if filt is not None:
+        kind, b, a = filt
+        mode = 1 if kind == "P1" else 2
+        A, B = _filter(mode, A, b, a), _filter(mode, B, b, a)
+        V, U = filter_series_host(V, b, a, mode), filter_series_host(U, b, a, mode)
+    return scatter2kernel(V, U, A, B, nt1, nt2, dt, M0)

@@ -20,6 +20,7 @@
 #ifndef FWANT_H
 #define FWANT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -166,8 +167,18 @@ int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, 
                        float *Kbq, void *stream);
 /* scatter_rotate, SI_ker_sta.F90:1354-1373 (rot: HOST 3x3 column-major). In place. */
 int fwk_rotate(long n, float *X, float *Y, float *Z, const float *rot, void *stream);
-/* Last error of the fwk_* family. */
+/* Last error of the fwk_* family and of the device-memory helpers below. */
 const char *fwk_last_error(void);
+
+/* Device-memory helpers for hosts without their own HIP allocator (the
+ * reference keeps these arrays in Fortran allocatables; a Fortran or Python
+ * caller of the fwk_* functions uses these to stage them in HBM). */
+int fw_device_alloc(size_t bytes, void **dptr);
+int fw_device_free(void *dptr);
+int fw_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int fw_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int fw_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int fw_device_sync(void);
 
 #ifdef __cplusplus
 }
