@@ -185,7 +185,8 @@ def gradient_400x400x200(nt=3000, ni=400, nj=400, nk=200, h=500.0, seed=3):
               abs_velocity=(8400.0,) * 6, cfs_bmax=(3.0,) * 6, cfs_amax=(np.pi * f0,) * 6)
     m.force = surface_force(ni, nj, nk, LENFD + ni // 2, LENFD + nj // 2, f0=f0)
     rng = np.random.default_rng(5)
-    ij = rng.integers(LENFD + 20, LENFD + ni - 20, size=(32, 2))
+    mg = min(20, ni // 4, nj // 4)
+    ij = rng.integers(LENFD + mg, LENFD + min(ni, nj) - mg, size=(32, 2))
     m.rec_indx = np.stack([ij[:, 0] + 1, ij[:, 1] + 1, np.full(32, m.nk2 + 1)], 1).astype(np.int32)
     m.rec_tinv = 1
     return m
