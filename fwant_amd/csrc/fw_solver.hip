@@ -52,7 +52,8 @@ thread_local std::string g_create_err;
 // stage-kernel tile shapes (threads 64 x BYT, RY rows per thread): selected
 // per handle (FWANT_TILE or the autotuner), all 16-row tiles
 constexpr int NTILECFG = 2;
-constexpr int TILE_RY[NTILECFG] = {2, 4};  // 8-row and 16-row tiles of 256 threads
+constexpr int TILE_BYT[NTILECFG] = {4, 8};  // waves per block
+constexpr int TILE_RY[NTILECFG] = {2, 2};   // rows per thread: 8-row and 16-row tiles
 
 // ---------------------------------------------------------------------------
 // Free-surface stress imaging (free_timg, mod_macdrp.F90:3414-3436) on the
@@ -302,10 +303,8 @@ struct fw_solver_s {
   float* d_msurf = nullptr;
   int* d_fluidcol = nullptr;  // per (i, j) column: 1 if any cell has mu <= SEIS_ZERO
   int tilecfg = 0;            // stage-kernel tile shape index
-  int4* d_items = nullptr;    // stage work items: interior items, then PML-shell items
+  int4* d_items = nullptr;    // stage work items: PML-shell items, then interior items
   int n_lean = 0, n_pml = 0;
-  hipStream_t stream2 = nullptr;  // shell items run here, concurrently with the interior
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // host copies
   std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
   std::vector<float> hlam, hmu, hrho;
@@ -383,12 +382,6 @@ int fw_create(const fw_config* cfg, fw_handle* out) {
   if (cfg->abs_number[5] > 0) h->freenode = 0;  // mod_abs_npml.F90:182-184
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete h; return fail(nullptr, FW_ERR_HIP, "hipStreamCreate failed"); }
-  if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess) {
-    fw_destroy(h);
-    return fail(nullptr, FW_ERR_HIP, "hipStreamCreate/hipEventCreate failed");
-  }
   const size_t setbytes = sizeof(float) * (size_t)(9 * h->fs);
   e = hipMalloc(&h->d_sets, 4 * setbytes + 64 * sizeof(float));
   if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc wavefield sets failed"); }
@@ -417,11 +410,7 @@ int fw_destroy(fw_handle h) {
   hipFree(h->d_pt); hipFree(h->d_rec);
   for (auto& s : h->snaps) hipFree(s.d);
   for (auto& e : h->ev) hipEventDestroy(e);
-  if (h->stream2) hipStreamSynchronize(h->stream2);
   hipFree(h->d_items);
-  if (h->ev_fork) hipEventDestroy(h->ev_fork);
-  if (h->ev_join) hipEventDestroy(h->ev_join);
-  if (h->stream2) hipStreamDestroy(h->stream2);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return FW_OK;
@@ -889,35 +878,33 @@ int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
 // ---------------------------------------------------------------------------
 typedef void (*StageFn)(StageArgs);
 
-template <int P, int K, int C, bool M>
+template <int P, int K, int C>
 static StageFn vel_fn() {
-  return k_vel<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C], M>;
+  return k_vel<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_BYT[C], TILE_RY[C]>;
 }
-template <int P, int K, int C, bool M>
+template <int P, int K, int C>
 static StageFn str_fn() {
-  return k_str<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_RY[C], M>;
+  return k_str<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_BYT[C], TILE_RY[C]>;
 }
-#define VROW(P, C, M) {vel_fn<P, 0, C, M>(), vel_fn<P, 1, C, M>(), vel_fn<P, 2, C, M>()}
-#define SROW(P, C, M) {str_fn<P, 0, C, M>(), str_fn<P, 1, C, M>(), str_fn<P, 2, C, M>()}
-#define VTAB(C, M) {VROW(0, C, M), VROW(1, C, M), VROW(2, C, M), VROW(3, C, M), \
-                    VROW(4, C, M), VROW(5, C, M), VROW(6, C, M), VROW(7, C, M)}
-#define STAB(C, M) {SROW(0, C, M), SROW(1, C, M), SROW(2, C, M), SROW(3, C, M), \
-                    SROW(4, C, M), SROW(5, C, M), SROW(6, C, M), SROW(7, C, M)}
-static StageFn vel_table(int c, int pml, int p, int kind) {
-  static const StageFn T[NTILECFG][2][8][3] = {{VTAB(0, false), VTAB(0, true)}, {VTAB(1, false), VTAB(1, true)}};
-  return T[c][pml][p][kind];
+#define VROW(P, C) {vel_fn<P, 0, C>(), vel_fn<P, 1, C>(), vel_fn<P, 2, C>()}
+#define SROW(P, C) {str_fn<P, 0, C>(), str_fn<P, 1, C>(), str_fn<P, 2, C>()}
+#define VTAB(C) {VROW(0, C), VROW(1, C), VROW(2, C), VROW(3, C), VROW(4, C), VROW(5, C), VROW(6, C), VROW(7, C)}
+#define STAB(C) {SROW(0, C), SROW(1, C), SROW(2, C), SROW(3, C), SROW(4, C), SROW(5, C), SROW(6, C), SROW(7, C)}
+static StageFn vel_table(int c, int p, int kind) {
+  static const StageFn T[NTILECFG][8][3] = {VTAB(0), VTAB(1)};
+  return T[c][p][kind];
 }
-static StageFn str_table(int c, int pml, int p, int kind) {
-  static const StageFn T[NTILECFG][2][8][3] = {{STAB(0, false), STAB(0, true)}, {STAB(1, false), STAB(1, true)}};
-  return T[c][pml][p][kind];
+static StageFn str_table(int c, int p, int kind) {
+  static const StageFn T[NTILECFG][8][3] = {STAB(0), STAB(1)};
+  return T[c][p][kind];
 }
 
 // Work items of the stage kernels: TX x TY tiles over chunks of k planes,
-// split into items that touch a CFS-PML slab (run by the PML-specialised
-// kernels) and interior items (lean kernels).  Each list is ordered by
-// chunk, tile row, tile column.
+// split into items that touch a CFS-PML slab (PML body) and interior items
+// (lean body).  Stored PML items first; each part ordered by chunk, tile
+// row, tile column.
 static int build_items(fw_handle h) {
-  const int TY = BYT * TILE_RY[h->tilecfg];
+  const int TY = TILE_BYT[h->tilecfg] * TILE_RY[h->tilecfg];
   const int ntx = (h->cfg.ni + TX - 1) / TX, nty = (h->cfg.nj + TY - 1) / TY;
   const PmlFace* F = h->face;
   const int zlo = F[4].on ? F[4].hi[2] : h->nk1 - 1;  // last plane of the z1 slab
@@ -954,8 +941,8 @@ static int build_items(fw_handle h) {
   h->d_items = nullptr;
   h->n_lean = (int)lean.size();
   h->n_pml = (int)pml.size();
-  std::vector<int4> all(lean);
-  all.insert(all.end(), pml.begin(), pml.end());
+  std::vector<int4> all(pml);
+  all.insert(all.end(), lean.begin(), lean.end());
   if (!all.empty()) {
     HIPCHK(hipMalloc(&h->d_items, sizeof(int4) * all.size()));
     HIPCHK(hipMemcpy(h->d_items, all.data(), sizeof(int4) * all.size(), hipMemcpyHostToDevice));
@@ -1017,25 +1004,15 @@ static int launch_stage(fw_handle h, int ntime, int st) {
     fa.ex0 = f.ex[0]; fa.ex1 = f.ex[1];
   }
   const int tc = h->tilecfg;
-  const dim3 sblk(TX, BYT);
+  const dim3 sblk(TX, TILE_BYT[tc]);
   if (h->timing) hipEventRecord(h->ev[2 * h->nstage], h->stream);
-  if (h->n_pml > 0) {  // shell items on the second stream, concurrent with the interior
-    hipEventRecord(h->ev_fork, h->stream);
-    hipStreamWaitEvent(h->stream2, h->ev_fork, 0);
-    StageArgs gp = g;
-    gp.items = h->d_items + h->n_lean;
-    gp.nitems = h->n_pml;
-    hipLaunchKernelGGL(vel_table(tc, 1, pat, kind), dim3(h->n_pml), sblk, 0, h->stream2, gp);
-    hipLaunchKernelGGL(str_table(tc, 1, pat, kind), dim3(h->n_pml), sblk, 0, h->stream2, gp);
-    hipEventRecord(h->ev_join, h->stream2);
+  g.items = h->d_items;
+  g.nitems = h->n_pml + h->n_lean;
+  g.npml = h->n_pml;
+  if (g.nitems > 0) {
+    hipLaunchKernelGGL(vel_table(tc, pat, kind), dim3(g.nitems), sblk, 0, h->stream, g);
+    hipLaunchKernelGGL(str_table(tc, pat, kind), dim3(g.nitems), sblk, 0, h->stream, g);
   }
-  if (h->n_lean > 0) {
-    g.items = h->d_items;
-    g.nitems = h->n_lean;
-    hipLaunchKernelGGL(vel_table(tc, 0, pat, kind), dim3(h->n_lean), sblk, 0, h->stream, g);
-    hipLaunchKernelGGL(str_table(tc, 0, pat, kind), dim3(h->n_lean), sblk, 0, h->stream, g);
-  }
-  if (h->n_pml > 0) hipStreamWaitEvent(h->stream, h->ev_join, 0);
   if (h->timing) {
     hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
     h->nstage++;

@@ -55,7 +55,7 @@ struct StageArgs {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   int freenode;
   const int4* __restrict__ items;  // work items: (i0, j0, k0, k1) of a tile over a chunk of planes
-  int nitems;
+  int nitems, npml;
   float a, b;
   // CFS-PML: per-axis profiles 1/B, A + D/B, D/B (x: nx, y: ny, z: nz entries)
   const float* __restrict__ pb1;
@@ -195,13 +195,15 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
 
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD
 // receives a contiguous run of work items (items are ordered by chunk, then
-// tile row, then tile column, so shared halos stay in one L2).
-__device__ __forceinline__ int4 item_of_block(const StageArgs& g) {
-  const int nb = g.nitems;
+// tile row, then tile column, so shared halos stay in one L2).  The list
+// holds the npml PML-shell items first (dispatched first: they are the
+// heavier ones), then the interior items; each part is remapped on its own.
+__device__ __forceinline__ int item_index(const StageArgs& g) {
   const int b = blockIdx.x;
-  const int q = nb / 8;
-  const int L = (b < 8 * q) ? (b % 8) * q + b / 8 : b;
-  return g.items[L];
+  const int base = b < g.npml ? 0 : g.npml;
+  const int n = b < g.npml ? g.npml : g.nitems - g.npml;
+  const int bb = b - base, q = n / 8;
+  return base + ((bb < 8 * q) ? (bb % 8) * q + bb / 8 : bb);
 }
 
 // one-sided 5-tap operator on LDS: p points at the centre, s is the stride
@@ -267,7 +269,8 @@ __device__ __forceinline__ HaloSlot xhalo_slot(const StageArgs& g, int i0, int j
 template <int YF, int TY, int LW>
 __device__ __forceinline__ HaloSlot yhalo_slot(const StageArgs& g, int i0, int j0, int tid) {
   constexpr int YL = YF ? 1 : 3;
-  HaloSlot h;
+  HaloSlot h{0u, -1, false};
+  if (tid >= 4 * TX) return h;
   const int c = tid % TX, s = tid / TX;
   const int row = s < YL ? j0 - YL + s : j0 + TY + (s - YL);
   const int col = i0 + c;
@@ -280,7 +283,6 @@ __device__ __forceinline__ float ld(const float* __restrict__ p, const HaloSlot&
   return h.ok ? p[h.off + kz] : 0.f;
 }
 
-constexpr int NTH = 256, BYT = 4;
 
 // ---------------------------------------------------------------------------
 // k_vel: momentum equations with the spherical geometric terms
@@ -288,12 +290,15 @@ constexpr int NTH = 256, BYT = 4;
 // (:917-926), RK update of Vx Vy Vz.
 // LDS: Txx (x), Txy (x, y), Txz (x), Tyy (y), Tyz (y); z window: Tzz Txz Tyz.
 // ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int KIND, int RY, bool PML>
-__global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
+template <int BYT, int RY>
+struct VelLds {
+  static constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, NL = 5, SIZE = 2 * NL * PL;
+};
+template <int XF, int YF, int ZF, int KIND, int BYT, int RY, bool PML>
+__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, float* shbase) {
   constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, NL = 5;
-  __shared__ float sh[2][NL * PL];  // Txx Txy Txz Tyy Tyz, double buffered
+  float(*sh)[NL * PL] = reinterpret_cast<float(*)[NL * PL]>(shbase);  // Txx Txy Txz Tyy Tyz, double buffered
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-  const int4 it = item_of_block(g);
   const int i0 = it.x, j0 = it.y, k0 = it.z, k1 = it.w;
   const bool tile_pml = PML && (g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && i0 + TX - 1 >= g.face[1].lo0) ||
                         (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
@@ -363,9 +368,11 @@ __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
       sTxy[hx.lds] = hxTxy;
       sTxz[hx.lds] = hxTxz;
     }
-    sTxy[hy.lds] = hyTxy;
-    sTyy[hy.lds] = hyTyy;
-    sTyz[hy.lds] = hyTyz;
+    if (hy.lds >= 0) {
+      sTxy[hy.lds] = hyTxy;
+      sTyy[hy.lds] = hyTyy;
+      sTyz[hy.lds] = hyTyz;
+    }
     // RK operands of plane k
     float rr[RY], mo[RY][3], to[RY][3];
     bool fl[RY];
@@ -467,12 +474,15 @@ __global__ __launch_bounds__(NTH) void k_vel(StageArgs g) {
 // Tzz Txz Tyz are the pre-image stresses saved by k_surface.
 // LDS: Vx Vy Vz (x and y halos); z window: Vx Vy Vz.
 // ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int KIND, int RY, bool PML>
-__global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
+template <int BYT, int RY>
+struct StrLds {
+  static constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, SIZE = 2 * 3 * PL;
+};
+template <int XF, int YF, int ZF, int KIND, int BYT, int RY, bool PML>
+__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, float* shbase) {
   constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH;
-  __shared__ float sh[2][3 * PL];  // Vx Vy Vz, double buffered
+  float(*sh)[3 * PL] = reinterpret_cast<float(*)[3 * PL]>(shbase);  // Vx Vy Vz, double buffered
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-  const int4 it = item_of_block(g);
   const int i0 = it.x, j0 = it.y, k0 = it.z, k1 = it.w;
   const bool tile_pml = PML && (g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && i0 + TX - 1 >= g.face[1].lo0) ||
                         (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
@@ -527,9 +537,11 @@ __global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
       s[PL + hx.lds] = hx1;
       s[2 * PL + hx.lds] = hx2;
     }
-    s[hy.lds] = hy0;
-    s[PL + hy.lds] = hy1;
-    s[2 * PL + hy.lds] = hy2;
+    if (hy.lds >= 0) {
+      s[hy.lds] = hy0;
+      s[PL + hy.lds] = hy1;
+      s[2 * PL + hy.lds] = hy2;
+    }
     const bool surf = g.freenode && k == g.nk2;
     // RK operands of plane k
     float la[RY], mu_[RY], mo[RY][6], to[RY][6];
@@ -630,6 +642,29 @@ __global__ __launch_bounds__(NTH) void k_str(StageArgs g) {
     hx0 = nx0; hx1 = nx1; hx2 = nx2;
     hy0 = ny0; hy1 = ny1; hy2 = ny2;
   }
+}
+
+// One launch per stage and kind: the PML-shell items run the PML body, the
+// interior items the lean body (each keeps its own register allocation).
+template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
+__global__ __launch_bounds__(TX* BYT) void k_vel(StageArgs g) {
+  __shared__ float sh[VelLds<BYT, RY>::SIZE];
+  const int L = item_index(g);
+  const int4 it = g.items[L];
+  if (L < g.npml)
+    vel_body<XF, YF, ZF, KIND, BYT, RY, true>(g, it, sh);
+  else
+    vel_body<XF, YF, ZF, KIND, BYT, RY, false>(g, it, sh);
+}
+template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
+__global__ __launch_bounds__(TX* BYT) void k_str(StageArgs g) {
+  __shared__ float sh[StrLds<BYT, RY>::SIZE];
+  const int L = item_index(g);
+  const int4 it = g.items[L];
+  if (L < g.npml)
+    str_body<XF, YF, ZF, KIND, BYT, RY, true>(g, it, sh);
+  else
+    str_body<XF, YF, ZF, KIND, BYT, RY, false>(g, it, sh);
 }
 
 }  // namespace fw
