@@ -9,10 +9,14 @@ resident in HBM before the timed region starts.
 
 Prints ONE JSON line:
   value      cell-updates/s (interior cells x steps / wall time of K steps)
-  roofline   the fused stage kernel: algorithmic bytes per launch (156 B per
-             cell, the average of 120/192/192/120 B over the 4 RK stages)
-             over its mean launch time measured with HIP events on the
-             solver's stream, against the 8 TB/s HBM peak
+  roofline   one RK stage (k_vel then k_str over all cells, the dominant
+             work): algorithmic bytes per stage (156 B per cell, the average
+             of 120/192/192/120 B over the 4 RK stages) over its mean
+             duration measured with HIP events on the solver's stream,
+             against the 8 TB/s HBM peak; traffic = HBM bytes per stage
+             from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same
+             counters on a plain copy (summarised by --summarize-pmc into
+             profiles/stage_traffic_gradient_400x400x200.json)
   cpu_baseline  the FP32 C oracle (OpenMP) on the host cores, timed on a
              bounded sample of the same workload (2 full steps)
 """
@@ -28,6 +32,72 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 624.0           # 156 FP32 values per cell per time step
 BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_UPDATE / 4
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "stage_traffic_gradient_400x400x200.json")
+COPY_BYTES = 1 << 30                    # --copy-calibration moves 1 GiB each way per copy
+
+
+def _kernel_family(name):
+    import re
+    m = re.search(r"\b(k_\w+)", name)
+    return m.group(1) if m else ("copy" if "copyBuffer" in name else name[:40])
+
+
+def _pmc_means(db):
+    """Mean counter value (KiB) per dispatch, by kernel family, from a rocprofv3 results database."""
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, dispatch_id, sum(value) from counters_collection "
+                       "group by dispatch_id").fetchall()
+    acc = {}
+    for name, _, v in rows:
+        acc.setdefault(_kernel_family(name), []).append(v)
+    return {k: sum(v) / len(v) for k, v in acc.items()}, {k: len(v) for k, v in acc.items()}
+
+
+def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, out):
+    """HBM bytes per RK stage (k_vel + k_str) from FETCH_SIZE / WRITE_SIZE runs.
+    Each counter is scaled by bytes-moved / counter-value of a plain 1 GiB device copy."""
+    f, nf = _pmc_means(fetch_db)
+    w, nw = _pmc_means(write_db)
+    cf, _ = _pmc_means(copy_fetch_db)
+    cw, _ = _pmc_means(copy_write_db)
+    ff = COPY_BYTES / (cf["copy"] * 1024.0)
+    fw = COPY_BYTES / (cw["copy"] * 1024.0)
+    kern = {}
+    for k in ("k_vel", "k_str"):
+        kern[k] = {"fetch_bytes": f[k] * 1024.0 * ff, "write_bytes": w[k] * 1024.0 * fw,
+                   "dispatches": nf[k]}
+    total = sum(v["fetch_bytes"] + v["write_bytes"] for v in kern.values())
+    res = {"bytes_per_launch": total, "launch": "one RK stage = k_vel + k_str over all cells",
+           "fetch_factor": ff, "write_factor": fw, "kernels": kern}
+    with open(out, "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(json.dumps(res))
+
+
+def summarize_trace(db, ncell):
+    """Markdown table of kernel durations by family (and by RK kind for the stage kernels)."""
+    import re
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, duration from kernels").fetchall()
+    acc = {}
+    for name, d in rows:
+        key = _kernel_family(name)
+        m = re.search(r"(k_vel|k_str)<([^>]*)>", name)
+        if m:
+            key += f" kind {m.group(2).split(',')[3].strip()}"
+        acc.setdefault(key, []).append(d / 1000.0)
+    tot = sum(sum(v) for v in acc.values())
+    print("| kernel | calls | total us | mean us | share |\n|---|---|---|---|---|")
+    for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        print(f"| {k} | {len(v)} | {sum(v):.0f} | {sum(v) / len(v):.1f} | {100 * sum(v) / tot:.1f}% |")
+    st = [sum(acc.get(k, [])) for k in acc if k.startswith("k_vel") or k.startswith("k_str")]
+    nst = len([1 for k, v in acc.items() if k.startswith("k_vel") for _ in v])
+    if nst:
+        mean = sum(st) / nst
+        print(f"\nmean stage (k_vel + k_str) {mean:.1f} us -> "
+              f"{ncell * BYTES_PER_CELL_LAUNCH / (mean * 1e-6) / 1e9:.0f} GB/s algorithmic")
 
 
 def log(*a):
@@ -44,11 +114,21 @@ def main():
     ap.add_argument("--nk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--traffic", type=str, default=None,
-                    help="JSON file with measured HBM bytes per stage-kernel launch")
+    ap.add_argument("--traffic", type=str, default=TRAFFIC_JSON,
+                    help="JSON file with measured HBM bytes per RK stage (written by --summarize-pmc)")
+    ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "OUT"),
+                    help="write the traffic JSON from rocprofv3 --pmc results databases and exit")
+    ap.add_argument("--summarize-trace", metavar="DB",
+                    help="print a kernel-duration table from a rocprofv3 --kernel-trace database and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.ni * args.nj * args.nk)
+        return
     if args.copy_calibration:
         import numpy as np
         from fwant_amd._lib import DeviceArray, device_sync
@@ -93,10 +173,10 @@ def main():
     s.timing(False)
     avg_ms = stage_ms / max(nst, 1)
     achieved = ncell * BYTES_PER_CELL_LAUNCH / (avg_ms * 1e-3) / 1e9
-    log(f"stage kernel: {nst} launches, mean {avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic; "
+    log(f"RK stage (k_vel + k_str): {nst} stages, mean {avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic; "
         f"stage kernels {stage_ms / total_ms * 100:.1f}% of step time")
     traffic = None
-    if args.traffic and os.path.exists(args.traffic):
+    if args.traffic and os.path.exists(args.traffic) and (args.ni, args.nj, args.nk) == (400, 400, 200):
         with open(args.traffic) as f:
             traffic = json.load(f).get("bytes_per_launch")
     s.close()
@@ -136,7 +216,7 @@ def main():
                    "hbm_fraction_whole_step": value * BYTES_PER_CELL_UPDATE / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stage (fused operator sweep + RK update)",
+                     "kernel": "RK stage = k_vel + k_str (operator sweep fused with the RK update)",
                      "mean_launch_us": avg_ms * 1e3},
         "cpu_baseline": cpu,
     }

@@ -24,6 +24,27 @@ namespace fw {
 
 constexpr int TX = 64;
 
+// Streamed operands (RK registers m and t, the stage output, the media) are
+// touched once per stage: non-temporal accesses keep them from evicting the
+// stencil planes whose halos neighbouring tiles re-read from L2.
+#ifndef FW_STREAM_NT
+#define FW_STREAM_NT 1
+#endif
+__device__ __forceinline__ float lds_nt(const float* p) {
+#if FW_STREAM_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void st_nt(float* p, float v) {
+#if FW_STREAM_NT
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 // One CFS-PML face slab (x1 x2 y1 y2 z1 z2): inclusive bounds, extents and
 // the auxiliary variables, 4 register sets (A B C T) x 6 variables x cnt.
 // Variable order: x: Vx Vy Vz Txx Txy Txz; y: Vx Vy Vz Tyy Txy Tyz;
@@ -379,13 +400,13 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const unsigned c = cc[q] + kz;
-      rr[q] = comp[q] ? g.rrho[c] : 0.f;
+      rr[q] = comp[q] ? lds_nt(g.rrho + c) : 0.f;
       fl[q] = fluid[q] ? g.mu[c] <= SEIS_ZERO : false;
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         const long e = f * fs + c;
-        mo[q][f] = (comp[q] && KIND != 2) ? (KIND == 0 ? W[e] : g.m[e]) : 0.f;
-        to[q][f] = (comp[q] && KIND != 0) ? g.t[e] : 0.f;
+        mo[q][f] = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : g.m + e) : 0.f;
+        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(g.t + e) : 0.f;
       }
     }
     // prefetch plane k+1
@@ -443,10 +464,10 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
           if constexpr (KIND == 2) {
-            g.wout[e] = to[q][f] + g.b * h[f];
+            st_nt(g.wout + e, to[q][f] + g.b * h[f]);
           } else {
-            g.wout[e] = mo[q][f] + g.a * h[f];
-            g.t[e] = (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f];
+            st_nt(g.wout + e, mo[q][f] + g.a * h[f]);
+            st_nt(g.t + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
           }
         }
       }
@@ -548,16 +569,16 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const unsigned c = cc[q] + kz;
-      la[q] = comp[q] ? g.lam[c] : 0.f;
-      mu_[q] = comp[q] ? g.mu[c] : 0.f;
+      la[q] = comp[q] ? lds_nt(g.lam + c) : 0.f;
+      mu_[q] = comp[q] ? lds_nt(g.mu + c) : 0.f;
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
         const long e = (TXX + f) * fs + c;
-        float mv = (comp[q] && KIND != 2) ? (KIND == 0 ? W[e] : g.m[e]) : 0.f;
+        float mv = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : g.m + e) : 0.f;
         if (surf && comp[q] && KIND != 2 && (f == 2 || f == 4 || f == 5))
           mv = g.msurf[(f == 2 ? 0 : (f == 4 ? 1 : 2)) * g.ps + cc[q]];
         mo[q][f] = mv;
-        to[q][f] = (comp[q] && KIND != 0) ? g.t[e] : 0.f;
+        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(g.t + e) : 0.f;
       }
     }
     // prefetch plane k+1
@@ -625,10 +646,10 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
         for (int f = 0; f < 6; ++f) {
           const long e = (TXX + f) * fs + c;
           if constexpr (KIND == 2) {
-            g.wout[e] = to[q][f] + g.b * h[f];
+            st_nt(g.wout + e, to[q][f] + g.b * h[f]);
           } else {
-            g.wout[e] = mo[q][f] + g.a * h[f];
-            g.t[e] = (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f];
+            st_nt(g.wout + e, mo[q][f] + g.a * h[f]);
+            st_nt(g.t + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
           }
         }
       }
