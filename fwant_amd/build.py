@@ -25,6 +25,33 @@ FLAGS = [
 ]
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(FORTRAN, "fw_drive")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """Fortran driver over the ISO_C_BINDING module (fortran/fwant_c.f90),
+    linked against the in-tree libfwant.so (rpath $ORIGIN/..)."""
+    bdir = os.path.join(FORTRAN, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    obj = os.path.join(bdir, "fwant_c.o")
+    steps = [
+        [FLANG, "-O2", "-module-dir", bdir, "-c", os.path.join(FORTRAN, "fwant_c.f90"), "-o", obj],
+        [FLANG, "-O2", "-module-dir", bdir, os.path.join(FORTRAN, "fw_drive.f90"), obj, "-L" + HERE, "-lfwant",
+         "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("flang failed building the Fortran driver")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = True) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -41,3 +68,4 @@ def build(verbose: bool = True) -> str:
 
 if __name__ == "__main__":
     build()
+    build_fortran()

@@ -1,0 +1,72 @@
+"""The drop-in from Fortran: fwant_amd/fortran/fw_drive (flang, through the
+ISO_C_BINDING module fwant_c) runs a case written from a Model and must
+give exactly what the Python binding gives on the same inputs, and match
+the FP32 oracle to 1e-5 rel-L2."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fwant_amd import models
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "fwant_amd", "fortran", "fw_drive")
+needs_driver = pytest.mark.skipif(not os.path.exists(DRIVER), reason="Fortran driver not built")
+
+
+def write_case(m, nsteps, path):
+    """Case file of fw_drive (layout in fwant_amd/fortran/fw_drive.f90)."""
+    f, mo = m.force, m.moment
+    i32 = lambda a: np.ascontiguousarray(a, np.int32).tobytes()  # noqa: E731
+    f32 = lambda a: np.ascontiguousarray(a, np.float32).tobytes()  # noqa: E731
+    hdr = [0x544E5746, 1, m.ni, m.nj, m.nk, int(m.freenode), nsteps, f.n, f.ntw, f.stf_id,
+           mo.n, mo.ntw, mo.stf_id, len(m.rec_indx), m.rec_tinv]
+    with open(path, "wb") as fh:
+        fh.write(i32(hdr))
+        fh.write(f32([m.stept]) + i32(m.abs_number) + f32(m.abs_velocity) + f32(m.cfs_bmax) + f32(m.cfs_amax))
+        for a in (m.x, m.y, m.z):
+            fh.write(np.ascontiguousarray(a, np.float64).tobytes())
+        for a in (m.lam, m.mu, m.rho):
+            fh.write(np.asarray(a, np.float32).ravel(order="F").tobytes())
+        fh.write(i32(f.indx.ravel()) + f32(f.shift.ravel()) + f32(f.t0) + f32(f.stf_time) + f32(f.stf_freq))
+        for q in range(3):
+            fh.write(f32(f.comps[q]) if f.n > 0 else b"")
+        fh.write(i32(mo.indx.ravel()) + f32(mo.shift.ravel()) + f32(mo.t0) + f32(mo.stf_time) + f32(mo.stf_freq))
+        fh.write(f32(mo.comps.ravel()) if mo.n > 0 else b"")
+        fh.write(i32(m.rec_indx.ravel()))
+
+
+@needs_driver
+def test_driver_probe():
+    r = subprocess.run([DRIVER], capture_output=True, text=True, timeout=60)
+    assert "fw_reference_steps(1000) = 1008" in r.stdout
+    assert "fw_create status" in r.stdout
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_driver_matches_python_and_oracle(gpu_lib, tmp_path):
+    from fwant_amd.solver import Solver
+    from oracle.pyoracle import OracleSolver
+    n = 64
+    m = models.small_test(ni=36, nj=32, nk=26, nt=n, seed=31)
+    case, out = tmp_path / "case.bin", tmp_path / "rec.bin"
+    write_case(m, n, case)
+    print(f"fw_drive on small_test 36x32x26, {n} steps", flush=True)
+    r = subprocess.run([DRIVER, str(case), str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    rec = np.fromfile(out, np.float32).reshape(n, len(m.rec_indx), 9)
+    s = Solver(m)
+    s.run(n)
+    g = s.receivers()
+    metric = s.metric
+    s.close()
+    np.testing.assert_array_equal(rec, g)
+    o = OracleSolver(m, 32, metric=metric)
+    ref = o.run(n).astype(np.float64)
+    o.close()
+    e = np.linalg.norm(rec - ref) / np.linalg.norm(ref)
+    print(f"Fortran driver vs FP32 oracle: rel-L2 {e:.2e}")
+    assert e <= 1e-5
