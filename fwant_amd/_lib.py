@@ -20,12 +20,16 @@ SYMBOLS = (
     "fw_create", "fw_destroy", "fw_last_error", "fw_dims", "fw_set_grid", "fw_build_metric",
     "fw_set_media", "fw_set_force", "fw_set_moment", "fw_set_receivers", "fw_add_snapshot",
     "fw_init", "fw_get_pml_profiles", "fw_step", "fw_run", "fw_reference_steps", "fw_sync",
-    "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_get_wavefield",
+    "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_add_snapshot_packed",
+    "fw_get_snapshot_packed", "fw_snapshot_packed_device", "fwk_sgt_decode", "fw_get_wavefield",
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
     "fwk_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
     "fw_memcpy_d2d", "fw_device_sync",
 )
+
+
+SGT_FORMATS = {"fp32": 0, "bf16": 1, "fp16": 2}
 
 
 class FwConfig(C.Structure):
@@ -71,6 +75,11 @@ def _sig(lib):
         "fw_get_receivers": (C.c_int, [H, fp, ip]),
         "fw_get_snapshot": (C.c_int, [H, C.c_int, fp, ip]),
         "fw_snapshot_device": (C.c_int, [H, C.c_int, C.POINTER(vp), ip, C.POINTER(C.c_long)]),
+        "fw_add_snapshot_packed": (C.c_int, [H, ip, ip, ip, C.c_int, C.c_uint, C.c_int, C.c_int, ip]),
+        "fw_get_snapshot_packed": (C.c_int, [H, C.c_int, vp, fp, ip, C.POINTER(C.c_long)]),
+        "fw_snapshot_packed_device": (C.c_int, [H, C.c_int, C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(C.c_long),
+                                                C.POINTER(C.c_long)]),
+        "fwk_sgt_decode": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp]),
         "fw_get_wavefield": (C.c_int, [H, C.c_int, fp]),
         "fw_set_wavefield": (C.c_int, [H, C.c_int, fp]),
         "fw_enable_timing": (C.c_int, [H, C.c_int]),

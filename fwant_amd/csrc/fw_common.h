@@ -2,6 +2,7 @@
 // functions, field indices.  gfx950 only.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
 #include <stdint.h>
 
 namespace fw {
@@ -123,6 +124,37 @@ __host__ __device__ inline float stf_eval(int which, int id, float t, float t0, 
     case STF_DELTA: return (ntime == 0) ? 1.0f / stept : 0.f;
     default: return 0.f;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Reduced-precision SGT storage: a value x of a 16x16x16 block with largest
+// magnitude amax is stored as a 16-bit code of x/amax (BF16) or of
+// x*32768/amax (FP16), both rounded to nearest even; the block keeps the FP32
+// scale amax (BF16) or amax/32768 (FP16), and x is restored as code * scale.
+// ---------------------------------------------------------------------------
+enum { SGT_FP32 = 0, SGT_BF16 = 1, SGT_FP16 = 2 };
+constexpr int SGT_B = 16;  // block edge in snapshot points
+
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t c) { return __uint_as_float((uint32_t)c << 16); }
+__device__ __forceinline__ uint16_t fp16_rne(float x) { return __half_as_ushort(__float2half_rn(x)); }
+__device__ __forceinline__ float fp16_val(uint16_t c) { return __half2float(__ushort_as_half(c)); }
+// the factor applied before rounding, and the stored scale, for a block maximum amax
+__device__ __forceinline__ float sgt_inv(int fmt, float amax) {
+  if (!(amax > 0.f)) return 0.f;
+  return fmt == SGT_FP16 ? 32768.0f / amax : 1.0f / amax;
+}
+__device__ __forceinline__ float sgt_scale(int fmt, float amax) { return fmt == SGT_FP16 ? amax * (1.0f / 32768.0f) : amax; }
+__device__ __forceinline__ uint16_t sgt_code(int fmt, float x, float inv) {
+  return fmt == SGT_FP16 ? fp16_rne(x * inv) : bf16_rne(x * inv);
+}
+__device__ __forceinline__ float sgt_value(int fmt, uint16_t c, float scale) {
+  return (fmt == SGT_FP16 ? fp16_val(c) : bf16_val(c)) * scale;
 }
 
 }  // namespace fw

@@ -10,6 +10,11 @@
 #include <string>
 #include <vector>
 
+using fw::SGT_B;
+using fw::SGT_BF16;
+using fw::SGT_FP16;
+using fw::sgt_value;
+
 namespace {
 thread_local std::string g_kerr;
 int kfail(int code, const std::string& m) {
@@ -42,6 +47,19 @@ __global__ void k_stress2strain(float* __restrict__ E, long cs, long ts, long np
     e[4 * cs] = 0.5f * E3 * e[4 * cs];
     e[5 * cs] = 0.5f * E3 * e[5 * cs];
   }
+}
+
+// reduced-precision SGT restore: out[slab][p] = code * scale of p's 16^3 block
+__global__ void k_sgt_decode(int fmt, const uint16_t* __restrict__ codes, const float* __restrict__ scales, int c0, int c1,
+                             long npts, int nb0, int nb1, long nblk, float* __restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  const long slab = blockIdx.y;
+  const int a = (int)(p % c0);
+  const long r = p / c0;
+  const int b = (int)(r % c1), d = (int)(r / c1);
+  const long blk = a / SGT_B + (long)nb0 * (b / SGT_B + (long)nb1 * (d / SGT_B));
+  out[slab * npts + p] = sgt_value(fmt, codes[slab * npts + p], scales[slab * nblk + blk]);
 }
 
 // sgt2scatter3 for one receiver component (SI_ker_sta.F90:1290-1331).
@@ -216,6 +234,19 @@ unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
 extern "C" {
 
 const char* fwk_last_error(void) { return g_kerr.c_str(); }
+
+int fwk_sgt_decode(int format, const uint16_t* codes, const float* scales, int c0, int c1, int c2, long nslab, float* out,
+                   void* stream) {
+  if (!codes || !scales || !out || c0 <= 0 || c1 <= 0 || c2 <= 0 || nslab <= 0 || nslab > 65535 ||
+      (format != SGT_BF16 && format != SGT_FP16))
+    return kfail(FW_ERR_ARG, "fwk_sgt_decode: bad argument");
+  const long npts = (long)c0 * c1 * c2;
+  const int nb0 = (c0 + SGT_B - 1) / SGT_B, nb1 = (c1 + SGT_B - 1) / SGT_B, nb2 = (c2 + SGT_B - 1) / SGT_B;
+  hipLaunchKernelGGL(k_sgt_decode, dim3((unsigned)((npts + 255) / 256), (unsigned)nslab), dim3(256), 0,
+                     (hipStream_t)stream, format, codes, scales, c0, c1, npts, nb0, nb1, (long)nb0 * nb1 * nb2, out);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
 
 // device memory helpers for hosts without their own HIP allocator
 int fw_device_alloc(size_t bytes, void** dptr) {

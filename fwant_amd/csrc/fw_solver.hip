@@ -234,6 +234,52 @@ __global__ void k_snapshot(SnapArgs g) {
   for (int q = 0; q < g.nf; ++q) g.out[q * g.npts + p] = g.W[g.flist[q] * g.fs + c];
 }
 
+// snapshot capture into reduced-precision storage: one workgroup per 16^3
+// block of snapshot points and field; a thread gathers a column of 16
+// points, the block maximum sets the scale, codes keep the point order of
+// the FP32 snapshot ([field][point], point index i fastest).
+struct PackArgs {
+  const float* __restrict__ W;
+  uint16_t* __restrict__ codes;  // [nf][npts] of this record
+  float* __restrict__ scales;    // [nf][nblk] of this record
+  long fs, sy, sz, npts;
+  int s0, s1, s2, c0, c1, c2, t0, t1, t2, nb0, nb1, nblk;
+  int nf, fmt;
+  int flist[9];
+};
+__global__ __launch_bounds__(256) void k_snapshot_pack(PackArgs g) {
+  const int blk = blockIdx.x, q = blockIdx.y;
+  const int b0 = blk % g.nb0, r = blk / g.nb0, b1 = r % g.nb1, b2 = r / g.nb1;
+  const int a = b0 * SGT_B + (int)(threadIdx.x & 15), b = b1 * SGT_B + (int)(threadIdx.x >> 4);
+  const bool in_ab = a < g.c0 && b < g.c1;
+  const float* W = g.W + g.flist[q] * g.fs;
+  float v[SGT_B];
+  float m = 0.f;
+#pragma unroll
+  for (int d = 0; d < SGT_B; ++d) {
+    const int dd = b2 * SGT_B + d;
+    v[d] = 0.f;
+    if (in_ab && dd < g.c2) {
+      const long c = (g.s0 + a * g.t0) + g.sy * (long)(g.s1 + b * g.t1) + g.sz * (long)(g.s2 + dd * g.t2);
+      v[d] = W[c];
+      m = fmaxf(m, fabsf(v[d]));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  __shared__ float wm[4];
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  const float amax = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+  const float inv = sgt_inv(g.fmt, amax);
+  if (threadIdx.x == 0) g.scales[(long)q * g.nblk + blk] = sgt_scale(g.fmt, amax);
+  uint16_t* out = g.codes + (long)q * g.npts;
+#pragma unroll
+  for (int d = 0; d < SGT_B; ++d) {
+    const int dd = b2 * SGT_B + d;
+    if (in_ab && dd < g.c2) out[a + (long)g.c0 * (b + (long)g.c1 * dd)] = sgt_code(g.fmt, v[d], inv);
+  }
+}
+
 // FP64 metric construction (seis3d_metric.F90:82-112): sin/cot of colatitude
 // and the inverse of x_xi = (B.x + F.x)/2; ghost values copied from the edge.
 __global__ void k_metric(const double* __restrict__ g, int n, int lo, int hi, float* __restrict__ inv,
@@ -280,7 +326,11 @@ struct Snapshot {
   int subs[3], subc[3], subt[3], tinv, nf, flist[9];
   long npts;
   int max_records, nrec = 0;
-  float* d = nullptr;
+  float* d = nullptr;           // FP32 records [nrec][nf][npts]
+  int fmt = SGT_FP32;           // or BF16 / FP16 codes with per-block scales
+  int nb[3]{}, nblk = 0;
+  uint16_t* codes = nullptr;    // [nrec][nf][npts]
+  float* scales = nullptr;      // [nrec][nf][nblk]
 };
 
 struct fw_solver_s {
@@ -408,7 +458,7 @@ int fw_destroy(fw_handle h) {
   for (auto& f : h->face) hipFree(f.d);
   for (void* p : h->d_src_allocs) hipFree(p);
   hipFree(h->d_pt); hipFree(h->d_rec);
-  for (auto& s : h->snaps) hipFree(s.d);
+  for (auto& s : h->snaps) { hipFree(s.d); hipFree(s.codes); hipFree(s.scales); }
   for (auto& e : h->ev) hipEventDestroy(e);
   hipFree(h->d_items);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -592,27 +642,48 @@ int fw_set_receivers(fw_handle h, int n, int tinv, const int* indx, int max_reco
   return FW_OK;
 }
 
-int fw_add_snapshot(fw_handle h, const int* subs, const int* subc, const int* subt, int tinv, unsigned mask,
-                    int max_records, int* snap_id) {
+static int add_snapshot(fw_handle h, const int* subs, const int* subc, const int* subt, int tinv, unsigned mask,
+                        int max_records, int fmt, int* snap_id, const char* who) {
   if (!h || !subs || !subc || !subt || tinv <= 0 || max_records <= 0 || !(mask & 0x1ff) || !snap_id)
-    return fail(h, FW_ERR_ARG, "fw_add_snapshot: bad argument");
+    return fail(h, FW_ERR_ARG, std::string(who) + ": bad argument");
+  if (fmt != SGT_FP32 && fmt != SGT_BF16 && fmt != SGT_FP16)
+    return fail(h, FW_ERR_ARG, std::string(who) + ": format must be FW_SGT_FP32, FW_SGT_BF16 or FW_SGT_FP16");
   Snapshot s{};
   const int ext[3] = {h->nx, h->ny, h->nz};
   for (int d = 0; d < 3; ++d) {
     s.subs[d] = subs[d] - 1; s.subc[d] = subc[d]; s.subt[d] = subt[d];
     if (subc[d] <= 0 || subt[d] <= 0 || s.subs[d] < 0 || s.subs[d] + (subc[d] - 1) * subt[d] >= ext[d])
-      return fail(h, FW_ERR_ARG, "fw_add_snapshot: sub-volume outside the ghosted grid");
+      return fail(h, FW_ERR_ARG, std::string(who) + ": sub-volume outside the ghosted grid");
+    s.nb[d] = (subc[d] + SGT_B - 1) / SGT_B;
   }
   s.tinv = tinv;
   s.nf = 0;
   for (int f = 0; f < 9; ++f)
     if (mask & (1u << f)) s.flist[s.nf++] = f;
   s.npts = (long)subc[0] * subc[1] * subc[2];
+  s.nblk = s.nb[0] * s.nb[1] * s.nb[2];
   s.max_records = max_records;
-  HIPCHK(hipMalloc(&s.d, sizeof(float) * (size_t)s.nf * s.npts * max_records));
+  s.fmt = fmt;
+  const size_t nv = (size_t)s.nf * s.npts * max_records;
+  if (fmt == SGT_FP32) {
+    HIPCHK(hipMalloc(&s.d, sizeof(float) * nv));
+  } else {
+    HIPCHK(hipMalloc(&s.codes, sizeof(uint16_t) * nv));
+    HIPCHK(hipMalloc(&s.scales, sizeof(float) * (size_t)s.nf * s.nblk * max_records));
+  }
   h->snaps.push_back(s);
   *snap_id = (int)h->snaps.size() - 1;
   return FW_OK;
+}
+
+int fw_add_snapshot(fw_handle h, const int* subs, const int* subc, const int* subt, int tinv, unsigned mask,
+                    int max_records, int* snap_id) {
+  return add_snapshot(h, subs, subc, subt, tinv, mask, max_records, SGT_FP32, snap_id, "fw_add_snapshot");
+}
+
+int fw_add_snapshot_packed(fw_handle h, const int* subs, const int* subc, const int* subt, int tinv, unsigned mask,
+                           int max_records, int format, int* snap_id) {
+  return add_snapshot(h, subs, subc, subt, tinv, mask, max_records, format, snap_id, "fw_add_snapshot_packed");
 }
 
 // PML profiles, abs_init (mod_abs_npml.F90:135-300) with the helpers of
@@ -1043,6 +1114,21 @@ static int capture_outputs(fw_handle h, int t1) {
   for (auto& s : h->snaps) {
     if (t1 % s.tinv != 0) continue;
     if (s.nrec >= s.max_records) return fail(h, FW_ERR_STATE, "snapshot storage full (max_records)");
+    if (s.fmt != SGT_FP32) {
+      PackArgs a{};
+      a.W = h->set[0];
+      a.codes = s.codes + (size_t)s.nrec * s.nf * s.npts;
+      a.scales = s.scales + (size_t)s.nrec * s.nf * s.nblk;
+      a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = s.npts;
+      a.s0 = s.subs[0]; a.s1 = s.subs[1]; a.s2 = s.subs[2];
+      a.c0 = s.subc[0]; a.c1 = s.subc[1]; a.c2 = s.subc[2];
+      a.t0 = s.subt[0]; a.t1 = s.subt[1]; a.t2 = s.subt[2];
+      a.nb0 = s.nb[0]; a.nb1 = s.nb[1]; a.nblk = s.nblk; a.nf = s.nf; a.fmt = s.fmt;
+      for (int q = 0; q < s.nf; ++q) a.flist[q] = s.flist[q];
+      hipLaunchKernelGGL(k_snapshot_pack, dim3((unsigned)s.nblk, (unsigned)s.nf), dim3(256), 0, h->stream, a);
+      s.nrec++;
+      continue;
+    }
     SnapArgs a{};
     a.W = h->set[0]; a.out = s.d + (size_t)s.nrec * s.nf * s.npts;
     a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = s.npts;
@@ -1137,6 +1223,7 @@ int fw_get_receivers(fw_handle h, float* out, int* nrecords) {
 
 int fw_get_snapshot(fw_handle h, int id, float* out, int* nrecords) {
   if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_get_snapshot: bad id");
+  if (h->snaps[id].fmt != SGT_FP32) return fail(h, FW_ERR_ARG, "fw_get_snapshot: packed snapshot, use fw_get_snapshot_packed");
   HIPCHK(hipStreamSynchronize(h->stream));
   const Snapshot& s = h->snaps[id];
   if (nrecords) *nrecords = s.nrec;
@@ -1145,8 +1232,37 @@ int fw_get_snapshot(fw_handle h, int id, float* out, int* nrecords) {
   return FW_OK;
 }
 
+int fw_get_snapshot_packed(fw_handle h, int id, uint16_t* codes, float* scales, int* nrecords, long* nblocks) {
+  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_get_snapshot_packed: bad id");
+  const Snapshot& s = h->snaps[id];
+  if (s.fmt == SGT_FP32) return fail(h, FW_ERR_ARG, "fw_get_snapshot_packed: FP32 snapshot, use fw_get_snapshot");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (nrecords) *nrecords = s.nrec;
+  if (nblocks) *nblocks = s.nblk;
+  if (codes && s.nrec > 0)
+    HIPCHK(hipMemcpy(codes, s.codes, sizeof(uint16_t) * (size_t)s.nf * s.npts * s.nrec, hipMemcpyDeviceToHost));
+  if (scales && s.nrec > 0)
+    HIPCHK(hipMemcpy(scales, s.scales, sizeof(float) * (size_t)s.nf * s.nblk * s.nrec, hipMemcpyDeviceToHost));
+  return FW_OK;
+}
+
+int fw_snapshot_packed_device(fw_handle h, int id, uint16_t** codes, float** scales, int* nrecords, long* npoints,
+                              long* nblocks) {
+  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_snapshot_packed_device: bad id");
+  const Snapshot& s = h->snaps[id];
+  if (s.fmt == SGT_FP32) return fail(h, FW_ERR_ARG, "fw_snapshot_packed_device: FP32 snapshot, use fw_snapshot_device");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (codes) *codes = s.codes;
+  if (scales) *scales = s.scales;
+  if (nrecords) *nrecords = s.nrec;
+  if (npoints) *npoints = s.npts;
+  if (nblocks) *nblocks = s.nblk;
+  return FW_OK;
+}
+
 int fw_snapshot_device(fw_handle h, int id, float** dptr, int* nrecords, long* npoints) {
   if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_snapshot_device: bad id");
+  if (h->snaps[id].fmt != SGT_FP32) return fail(h, FW_ERR_ARG, "fw_snapshot_device: packed snapshot, use fw_snapshot_packed_device");
   HIPCHK(hipStreamSynchronize(h->stream));
   const Snapshot& s = h->snaps[id];
   if (dptr) *dptr = s.d;

@@ -12,6 +12,7 @@ the default stream; the synchronous copies of DeviceArray order it.
   scatter_filter(w0, b, a) / scatter_filtfilt   :1433-1475
   scatter2kernel(V, U, Sa, Sb, nt1, nt2, dt, M0)  :1375-1431
   scatter_rotate(X, Y, Z, rot)              :1354-1373
+  sgt_decode(fmt, codes, scales, subc, nslab, out)   packed BF16/FP16 SGT -> FP32
   pair_kernel(...)                          the per-block pipeline of :574-798
 """
 import ctypes as C
@@ -127,6 +128,15 @@ def filter_series_host(x, b, a, mode):
 
     y = iir(v) if mode == 1 else iir(iir(v)[::-1])[::-1]
     return y[nfact:nfact + knt].astype(np.float32)
+
+
+def sgt_decode(fmt, codes, scales, subc, nslab, out):
+    """Restore packed SGT slabs (fw_add_snapshot_packed layout) to FP32:
+    codes, scales and out are device pointers or DeviceArrays."""
+    f = _lib.SGT_FORMATS[fmt] if isinstance(fmt, str) else int(fmt)
+    p = lambda t: C.c_void_p(t) if isinstance(t, int) else _ptr(t)  # noqa: E731
+    kcheck(_lib.lib().fwk_sgt_decode(f, p(codes), p(scales), int(subc[0]), int(subc[1]), int(subc[2]), int(nslab),
+                                     p(out), _stream()), "fwk_sgt_decode")
 
 
 def pair_kernel(Rx, Ry, Rz, S, lam, mu, dt, V, U, nt1, nt2, M0, comp, filt=None):

@@ -23,7 +23,9 @@ class Solver:
         """model: fwant_amd.models.Model.  metric: None -> build on the GPU in
         FP64 (fw_build_metric); or a dict with xsin, xcot, xi_x, eta_y, zeta_z
         (float32) as read from metric_mpi*.nc.  snapshots: iterable of dicts
-        with subs, subc, subt (1-based), tinv, fields (names) and max_records."""
+        with subs, subc, subt (1-based), tinv, fields (names), max_records and
+        optionally format ("fp32", "bf16" or "fp16": 16-bit codes with a
+        per-16^3-block FP32 scale)."""
         L = _lib.lib()
         self.L = L
         self.model = model
@@ -94,9 +96,17 @@ class Solver:
             subs = np.array(s["subs"], np.int32)
             subc = np.array(s["subc"], np.int32)
             subt = np.array(s["subt"], np.int32)
-            check(L.fw_add_snapshot(h, iptr(subs), iptr(subc), iptr(subt), int(s["tinv"]), mask,
-                                    int(s["max_records"]), C.byref(sid)), h, "fw_add_snapshot")
-            self.snaps.append(dict(s, id=sid.value, nf=len(s["fields"]), npts=int(np.prod(subc))))
+            fmt = s.get("format", "fp32")
+            if fmt == "fp32":
+                check(L.fw_add_snapshot(h, iptr(subs), iptr(subc), iptr(subt), int(s["tinv"]), mask,
+                                        int(s["max_records"]), C.byref(sid)), h, "fw_add_snapshot")
+            else:
+                check(L.fw_add_snapshot_packed(h, iptr(subs), iptr(subc), iptr(subt), int(s["tinv"]), mask,
+                                               int(s["max_records"]), _lib.SGT_FORMATS[fmt], C.byref(sid)), h,
+                      "fw_add_snapshot_packed")
+            nblk = int(np.prod([(c + 15) // 16 for c in s["subc"]]))
+            self.snaps.append(dict(s, id=sid.value, nf=len(s["fields"]), npts=int(np.prod(subc)), nblk=nblk,
+                                   format=fmt))
         check(L.fw_init(h), h, "fw_init")
         self.ntime = 0
 
@@ -135,6 +145,29 @@ class Solver:
         check(self.L.fw_snapshot_device(self.h, s["id"], C.byref(p), C.byref(n), C.byref(npts)), self.h,
               "fw_snapshot_device")
         return p.value, n.value, npts.value
+
+    def snapshot_packed(self, k=0):
+        """codes [nrec][nf][npts] (uint16) and scales [nrec][nf][nblocks] of a packed snapshot."""
+        s = self.snaps[k]
+        n = C.c_int()
+        nb = C.c_long()
+        check(self.L.fw_get_snapshot_packed(self.h, s["id"], None, None, C.byref(n), C.byref(nb)), self.h,
+              "fw_get_snapshot_packed")
+        codes = np.zeros((n.value, s["nf"], s["npts"]), np.uint16)
+        scales = np.zeros((n.value, s["nf"], nb.value), np.float32)
+        if n.value:
+            check(self.L.fw_get_snapshot_packed(self.h, s["id"], codes.ctypes.data_as(C.c_void_p), fptr(scales),
+                                                C.byref(n), C.byref(nb)), self.h, "fw_get_snapshot_packed")
+        return codes, scales
+
+    def snapshot_packed_device(self, k=0):
+        """(codes pointer, scales pointer, records, points, blocks) of a packed snapshot in HBM."""
+        s = self.snaps[k]
+        pc, ps = C.c_void_p(), C.c_void_p()
+        n, npts, nb = C.c_int(), C.c_long(), C.c_long()
+        check(self.L.fw_snapshot_packed_device(self.h, s["id"], C.byref(pc), C.byref(ps), C.byref(n), C.byref(npts),
+                                               C.byref(nb)), self.h, "fw_snapshot_packed_device")
+        return pc.value, ps.value, n.value, npts.value, nb.value
 
     def wavefield(self, f):
         fi = _lib.FIELDS.index(f) if isinstance(f, str) else int(f)

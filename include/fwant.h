@@ -132,6 +132,23 @@ int fw_get_receivers(fw_handle h, float *out, int *nrecords);
  * the same storage (valid until fw_destroy) for zero-copy consumers. */
 int fw_get_snapshot(fw_handle h, int snap_id, float *out, int *nrecords);
 int fw_snapshot_device(fw_handle h, int snap_id, float **dptr, int *nrecords, long *npoints);
+
+/* Reduced-precision SGT storage (no counterpart in the reference, which
+ * writes FP32 NetCDF): like fw_add_snapshot, but each record of each field
+ * is stored as 16-bit codes (FW_SGT_BF16 or FW_SGT_FP16) with one FP32 scale
+ * per 16x16x16 block of snapshot points.  A value x of a block whose largest
+ * magnitude is amax is coded as BF16(x/amax) or FP16(x*32768/amax), rounded
+ * to nearest even, with scale amax or amax/32768; it is restored as
+ * code * scale.  codes[nrec][nfield][npts] keep the point order of the FP32
+ * snapshot; scales[nrec][nfield][nblocks], block index b0 + nb0*(b1 + nb1*b2)
+ * with nbd = ceil(subc[d]/16).  fw_get_snapshot / fw_snapshot_device reject
+ * packed snapshots and the packed calls reject FP32 ones. */
+enum { FW_SGT_FP32 = 0, FW_SGT_BF16 = 1, FW_SGT_FP16 = 2 };
+int fw_add_snapshot_packed(fw_handle h, const int *subs, const int *subc, const int *subt, int tinv,
+                           unsigned field_mask, int max_records, int format, int *snap_id);
+int fw_get_snapshot_packed(fw_handle h, int snap_id, uint16_t *codes, float *scales, int *nrecords, long *nblocks);
+int fw_snapshot_packed_device(fw_handle h, int snap_id, uint16_t **codes, float **scales, int *nrecords,
+                              long *npoints, long *nblocks);
 /* Wavefield state (between steps): ghosted nx*ny*nz host arrays.  Replaces
  * io_rest_export / io_rest_import's array transfer (mod_io.F90:342-427). */
 int fw_get_wavefield(fw_handle h, int field, float *out);
@@ -167,6 +184,11 @@ int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, 
                        float *Kbq, void *stream);
 /* scatter_rotate, SI_ker_sta.F90:1354-1373 (rot: HOST 3x3 column-major). In place. */
 int fwk_rotate(long n, float *X, float *Y, float *Z, const float *rot, void *stream);
+/* Restore packed SGT records (fw_add_snapshot_packed layout) to FP32:
+ * nslab = records x fields consecutive [npts] slabs of a c0 x c1 x c2
+ * snapshot, codes and scales on the device, out[nslab][npts]. */
+int fwk_sgt_decode(int format, const uint16_t *codes, const float *scales, int c0, int c1, int c2, long nslab,
+                   float *out, void *stream);
 /* Last error of the fwk_* family and of the device-memory helpers below. */
 const char *fwk_last_error(void);
 

@@ -1060,3 +1060,94 @@ void ORC(scatter_rotate)(long npt, int knt, real *X, real *Y, real *Z, const rea
     Z[q] = v1 * rot[6] + v2 * rot[7] + v3 * rot[8];
   }
 }
+
+/* ========================================================================
+ * Reduced-precision SGT storage (no reference counterpart: the reference
+ * stores FP32).  Per 16^3 block of snapshot points and per slab: amax =
+ * largest |x|; BF16 codes of x/amax (scale amax) or FP16 codes of
+ * x*32768/amax (scale amax/32768), round to nearest even; x' = code*scale.
+ * Always float arithmetic, whatever REAL is.  Pinned by
+ * tests/test_oracle_sgt_codec.py: conversions against numpy float16 and
+ * torch bfloat16, block scales against numpy maxima, restore error bounds.
+ * ====================================================================== */
+#include <stdint.h>
+static uint16_t c_bf16_rne(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static float c_bf16_val(uint16_t c) {
+  uint32_t u = (uint32_t)c << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+static uint16_t c_fp16_rne(float x) {
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);
+  if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* rounds to infinity */
+  const uint32_t e = a >> 23, m = (a & 0x7fffffu) | 0x800000u;
+  if (e < 113) { /* result subnormal (or zero): k units of 2^-24 */
+    if (e < 102) return (uint16_t)sign;
+    const uint32_t sh = 126 - e, k = m >> sh, rem = m & ((1u << sh) - 1), half = 1u << (sh - 1);
+    const uint32_t r = k + ((rem > half || (rem == half && (k & 1u))) ? 1u : 0u);
+    return (uint16_t)(sign | r);
+  }
+  const uint32_t k = ((e - 112) << 10) | ((a & 0x7fffffu) >> 13), rem = a & 0x1fffu;
+  const uint32_t r = k + ((rem > 0x1000u || (rem == 0x1000u && (k & 1u))) ? 1u : 0u);
+  return (uint16_t)(sign | r);
+}
+static float c_fp16_val(uint16_t c) {
+  const uint32_t sign = (uint32_t)(c & 0x8000u) << 16, e = (c >> 10) & 0x1fu, m = c & 0x3ffu;
+  float f;
+  if (e == 0) {
+    f = (float)m * 5.9604644775390625e-08f; /* m * 2^-24 */
+    return sign ? -f : f;
+  }
+  uint32_t u = sign | (e == 31 ? (0xffu << 23) | (m << 13) : ((e + 112) << 23) | (m << 13));
+  memcpy(&f, &u, 4);
+  return f;
+}
+void ORC(sgt_encode)(int fmt, int c0, int c1, int c2, long nslab, const float *x, uint16_t *codes, float *scales) {
+  const int nb0 = (c0 + 15) / 16, nb1 = (c1 + 15) / 16, nb2 = (c2 + 15) / 16;
+  const long npts = (long)c0 * c1 * c2, nblk = (long)nb0 * nb1 * nb2;
+  for (long sl = 0; sl < nslab; sl++)
+    for (int b2 = 0; b2 < nb2; b2++)
+      for (int b1 = 0; b1 < nb1; b1++)
+        for (int b0 = 0; b0 < nb0; b0++) {
+          const long blk = b0 + (long)nb0 * (b1 + (long)nb1 * b2);
+          float amax = 0.0f;
+          for (int d = b2 * 16; d < c2 && d < b2 * 16 + 16; d++)
+            for (int b = b1 * 16; b < c1 && b < b1 * 16 + 16; b++)
+              for (int a = b0 * 16; a < c0 && a < b0 * 16 + 16; a++)
+                amax = fmaxf(amax, fabsf(x[sl * npts + a + (long)c0 * (b + (long)c1 * d)]));
+          const float inv = amax > 0.0f ? (fmt == 2 ? 32768.0f / amax : 1.0f / amax) : 0.0f;
+          scales[sl * nblk + blk] = fmt == 2 ? amax * (1.0f / 32768.0f) : amax;
+          for (int d = b2 * 16; d < c2 && d < b2 * 16 + 16; d++)
+            for (int b = b1 * 16; b < c1 && b < b1 * 16 + 16; b++)
+              for (int a = b0 * 16; a < c0 && a < b0 * 16 + 16; a++) {
+                const long p = sl * npts + a + (long)c0 * (b + (long)c1 * d);
+                const float q = x[p] * inv;
+                codes[p] = fmt == 2 ? c_fp16_rne(q) : c_bf16_rne(q);
+              }
+        }
+}
+void ORC(sgt_decode)(int fmt, int c0, int c1, int c2, long nslab, const uint16_t *codes, const float *scales, float *out) {
+  const int nb0 = (c0 + 15) / 16, nb1 = (c1 + 15) / 16, nb2 = (c2 + 15) / 16;
+  const long npts = (long)c0 * c1 * c2, nblk = (long)nb0 * nb1 * nb2;
+  for (long sl = 0; sl < nslab; sl++)
+    for (long p = 0; p < npts; p++) {
+      const int a = (int)(p % c0), b = (int)((p / c0) % c1), d = (int)(p / ((long)c0 * c1));
+      const long blk = a / 16 + (long)nb0 * (b / 16 + (long)nb1 * (d / 16));
+      const uint16_t c = codes[sl * npts + p];
+      out[sl * npts + p] = (fmt == 2 ? c_fp16_val(c) : c_bf16_val(c)) * scales[sl * nblk + blk];
+    }
+}
+/* single-value conversions, exported for the codec pins */
+uint16_t ORC(f32_to_bf16)(float x) { return c_bf16_rne(x); }
+uint16_t ORC(f32_to_fp16)(float x) { return c_fp16_rne(x); }
+float ORC(fp16_to_f32)(uint16_t c) { return c_fp16_val(c); }

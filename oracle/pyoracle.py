@@ -57,6 +57,11 @@ def _load(prec):
         "series_filter": (None, [C.c_int, C.c_long, C.c_int, C.c_int, dp, dp, rp, rp]),
         "scatter2kernel": (None, [C.c_long, C.c_int, C.c_int, C.c_int, R, R, rp, rp, rp, rp, rp, rp, rp, rp]),
         "scatter_rotate": (None, [C.c_long, C.c_int, rp, rp, rp, rp]),
+        "sgt_encode": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp, vp]),
+        "sgt_decode": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp, vp]),
+        "f32_to_bf16": (C.c_uint16, [C.c_float]),
+        "f32_to_fp16": (C.c_uint16, [C.c_float]),
+        "fp16_to_f32": (C.c_float, [C.c_uint16]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, p + name)
@@ -87,6 +92,33 @@ class Lib:
 
     def arr(self, a):
         return np.ascontiguousarray(np.asarray(a, dtype=self.dtype).ravel(order="K"))
+
+
+def sgt_encode(fmt, x, subc):
+    """Oracle packing of FP32 SGT slabs x[..., npts] (fmt "bf16" | "fp16"):
+    returns codes (uint16, same shape) and scales [..., nblocks]."""
+    f = {"bf16": 1, "fp16": 2}[fmt]
+    x = np.ascontiguousarray(x, np.float32)
+    npts = int(np.prod(subc))
+    assert x.shape[-1] == npts
+    nslab = x.size // npts
+    nblk = int(np.prod([(c + 15) // 16 for c in subc]))
+    codes = np.zeros(x.shape, np.uint16)
+    scales = np.zeros(x.shape[:-1] + (nblk,), np.float32)
+    Lib(32).sgt_encode(f, int(subc[0]), int(subc[1]), int(subc[2]), nslab, x.ctypes.data, codes.ctypes.data,
+                       scales.ctypes.data)
+    return codes, scales
+
+
+def sgt_decode(fmt, codes, scales, subc):
+    f = {"bf16": 1, "fp16": 2}[fmt]
+    codes = np.ascontiguousarray(codes, np.uint16)
+    scales = np.ascontiguousarray(scales, np.float32)
+    npts = int(np.prod(subc))
+    out = np.zeros(codes.shape, np.float32)
+    Lib(32).sgt_decode(f, int(subc[0]), int(subc[1]), int(subc[2]), codes.size // npts, codes.ctypes.data,
+                       scales.ctypes.data, out.ctypes.data)
+    return out
 
 
 def build_metric(model):
