@@ -21,7 +21,8 @@ SYMBOLS = (
     "fw_set_media", "fw_set_force", "fw_set_moment", "fw_set_receivers", "fw_add_snapshot",
     "fw_init", "fw_get_pml_profiles", "fw_step", "fw_run", "fw_reference_steps", "fw_sync",
     "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_add_snapshot_packed",
-    "fw_get_snapshot_packed", "fw_snapshot_packed_device", "fwk_sgt_decode", "fw_get_wavefield",
+    "fw_get_snapshot_packed", "fw_snapshot_packed_device", "fwk_sgt_decode", "fw_set_batch",
+    "fw_select_member", "fw_set_batch_width", "fw_use_graphs", "fw_autotune", "fw_get_wavefield",
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
     "fwk_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
@@ -80,6 +81,11 @@ def _sig(lib):
         "fw_snapshot_packed_device": (C.c_int, [H, C.c_int, C.POINTER(vp), C.POINTER(vp), ip, C.POINTER(C.c_long),
                                                 C.POINTER(C.c_long)]),
         "fwk_sgt_decode": (C.c_int, [C.c_int, vp, vp, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp]),
+        "fw_set_batch": (C.c_int, [H, C.c_int, ip, ip, ip]),
+        "fw_select_member": (C.c_int, [H, C.c_int]),
+        "fw_set_batch_width": (C.c_int, [H, C.c_int]),
+        "fw_use_graphs": (C.c_int, [H, C.c_int]),
+        "fw_autotune": (C.c_int, [H, C.c_int, ip, ip, fp]),
         "fw_get_wavefield": (C.c_int, [H, C.c_int, fp]),
         "fw_set_wavefield": (C.c_int, [H, C.c_int, fp]),
         "fw_enable_timing": (C.c_int, [H, C.c_int]),

@@ -69,35 +69,40 @@ struct SurfEntry {  // one surface force point (the last (n, m) writing it wins)
 struct SurfArgs {
   float* win;
   float* msurf;
-  const int* __restrict__ smap;  // plane map -> entry or -1
+  const int* __restrict__ smap;  // plane map -> entry or -1 (one plane map per member)
   const SurfEntry* __restrict__ ent;
   const float* __restrict__ ft0;    // force start time [n]
   const float* __restrict__ ftime;  // window t0 [ntw]
   const float* __restrict__ ffreq;  // window f0 [ntw]
   const float* __restrict__ F;      // Fx Fy Fz blocks [3][ntw*n]
-  long fs, sy, sz, ps, fblk;
-  int ni1, ni2, nj1, nj2, nk2, ntw, stf_id, ntime, save;
-  float ts, stept;
+  long fs, sy, sz, ps, fblk, bstride;
+  const int* __restrict__ clk;  // device step clock: clk[0] = ntime
+  int ni1, ni2, nj1, nj2, nk2, ntw, stf_id, save;
+  float tinc, stept;
 };
 __global__ void k_surface(SurfArgs g) {
   const int i = g.ni1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = g.nj1 + blockIdx.y * blockDim.y + threadIdx.y;
   if (i > g.ni2 || j > g.nj2) return;
+  const long slot = blockIdx.z;
+  const int ntime = g.clk[0];
+  const float ts = ((float)ntime + g.tinc) * g.stept;
   const long c2 = i + g.sy * j;
   const long cs = c2 + g.sz * g.nk2;
-  float* Tzz = g.win + TZZ * g.fs;
-  float* Txz = g.win + TXZ * g.fs;
-  float* Tyz = g.win + TYZ * g.fs;
+  float* Tzz = g.win + slot * g.bstride + TZZ * g.fs;
+  float* Txz = g.win + slot * g.bstride + TXZ * g.fs;
+  float* Tyz = g.win + slot * g.bstride + TYZ * g.fs;
+  float* msurf = g.msurf + slot * 3 * g.ps;
   if (g.save) {
-    g.msurf[c2] = Tzz[cs];
-    g.msurf[g.ps + c2] = Txz[cs];
-    g.msurf[2 * g.ps + c2] = Tyz[cs];
+    msurf[c2] = Tzz[cs];
+    msurf[g.ps + c2] = Txz[cs];
+    msurf[2 * g.ps + c2] = Tyz[cs];
   }
   float tx = 0.f, ty = 0.f, tz = 0.f;
-  const int e = g.smap[c2];
+  const int e = g.smap[slot * g.ps + c2];
   if (e >= 0) {
     const SurfEntry se = g.ent[e];
-    const float disp = stf_eval(0, g.stf_id, g.ts - g.ft0[se.n], g.ftime[se.m], g.ffreq[se.m], g.ntime, g.stept);
+    const float disp = stf_eval(0, g.stf_id, ts - g.ft0[se.n], g.ftime[se.m], g.ffreq[se.m], ntime, g.stept);
     const long q = se.m + (long)g.ntw * se.n;
     tx = g.F[q] / se.S * disp;
     ty = g.F[g.fblk + q] / se.S * disp;
@@ -147,13 +152,19 @@ struct SrcArgs {
   const float* __restrict__ M;  // [6][ntw*n]
   long mblk;
   int mntw, mstf;
-  long fs;
-  int ncell, ntime, kind;
-  float ts, stept, a, b;
+  long fs, bstride;
+  const int* __restrict__ rowslot;  // member slot of each affected cell
+  const int* __restrict__ clk;      // device step clock: clk[0] = ntime
+  int ncell, nact, kind;
+  float tinc, stept, a, b;
 };
 __global__ void k_src(SrcArgs g) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= g.ncell) return;
+  const long slot = g.rowslot[u];
+  if (slot >= g.nact) return;  // member past its record length
+  const int ntime = g.clk[0];
+  const float ts = ((float)ntime + g.tinc) * g.stept;
   const long c = g.cell[u];
   float dh[9];
 #pragma unroll
@@ -162,7 +173,7 @@ __global__ void k_src(SrcArgs g) {
     const SrcContrib s = g.con[q];
     if (s.type == 0) {
       for (int m = 0; m < g.fntw; ++m) {
-        const float disp = stf_eval(0, g.fstf, g.ts - g.ft0[s.n], g.ftime[m], g.ffreq[m], g.ntime, g.stept);
+        const float disp = stf_eval(0, g.fstf, ts - g.ft0[s.n], g.ftime[m], g.ffreq[m], ntime, g.stept);
         const long qq = m + (long)g.fntw * s.n;
         dh[VX] += g.F[qq] * disp * s.w;
         dh[VY] += g.F[g.fblk + qq] * disp * s.w;
@@ -170,7 +181,7 @@ __global__ void k_src(SrcArgs g) {
       }
     } else if (s.type == 1) {
       for (int m = 0; m < g.mntw; ++m) {
-        const float rate = stf_eval(1, g.mstf, g.ts - g.mt0[s.n], g.mtime[m], g.mfreq[m], g.ntime, g.stept);
+        const float rate = stf_eval(1, g.mstf, ts - g.mt0[s.n], g.mtime[m], g.mfreq[m], ntime, g.stept);
         const float v = s.w * rate;
         const long qq = m + (long)g.mntw * s.n;
         dh[TXX] -= g.M[qq] * v;
@@ -181,7 +192,7 @@ __global__ void k_src(SrcArgs g) {
         dh[TYZ] -= g.M[5 * g.mblk + qq] * v;
       }
     } else {
-      const float rate = stf_eval(1, g.fstf, g.ts - g.ft0[s.n], g.ftime[s.m], g.ffreq[s.m], g.ntime, g.stept);
+      const float rate = stf_eval(1, g.fstf, ts - g.ft0[s.n], g.ftime[s.m], g.ffreq[s.m], ntime, g.stept);
       const long qq = s.m + (long)g.fntw * s.n;
       const float vx = g.F[qq] * s.w * rate, vy = g.F[g.fblk + qq] * s.w * rate,
                   vz = g.F[2 * g.fblk + qq] * s.w * rate;
@@ -194,44 +205,82 @@ __global__ void k_src(SrcArgs g) {
       dh[TYZ] += 2.0f * miu * dE23;
     }
   }
+  float* wout = g.wout + slot * g.bstride;
+  float* t = g.t + slot * g.bstride;
 #pragma unroll
   for (int f = 0; f < 9; ++f) {
     if (g.kind == 2) {
-      g.wout[f * g.fs + c] += g.b * dh[f];
+      wout[f * g.fs + c] += g.b * dh[f];
     } else {
-      g.wout[f * g.fs + c] += g.a * dh[f];
-      g.t[f * g.fs + c] += g.b * dh[f];
+      wout[f * g.fs + c] += g.a * dh[f];
+      t[f * g.fs + c] += g.b * dh[f];
     }
   }
 }
 
-// receivers: rec[record][p][9]
-__global__ void k_receivers(const float* __restrict__ W, long fs, const long* __restrict__ pt, int npt,
-                            float* __restrict__ rec) {
+// receivers after step ntime = clk[0] when (ntime+1) % tinv == 0: member
+// slot blockIdx.y writes record clk[1] of its rec[max_rec][npt][9]
+__global__ void k_receivers(const float* __restrict__ W, long fs, long bstride, const long* __restrict__ pt, int npt,
+                            float* __restrict__ rec, int max_rec, int tinv, const int* __restrict__ clk) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npt * 9) return;
+  if (q >= npt * 9 || (clk[0] + 1) % tinv != 0) return;
+  const int r = clk[1];
+  if (r >= max_rec) return;
+  const long slot = blockIdx.y;
   const int p = q / 9, f = q % 9;
-  rec[q] = W[f * fs + pt[p]];
+  rec[((size_t)slot * max_rec + r) * npt * 9 + q] = W[slot * bstride + f * fs + pt[p]];
+}
+
+// device step clock: after the outputs of step clk[0], advance the record
+// counters of the captures that happened and the step itself
+struct TickArgs {
+  int* clk;
+  int rec_tinv;  // 0: no receivers
+  int nsnap;
+  int tinv[16];
+};
+__global__ void k_tick(TickArgs g) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int t1 = g.clk[0] + 1;
+  if (g.rec_tinv > 0 && t1 % g.rec_tinv == 0) g.clk[1] += 1;
+  for (int s = 0; s < g.nsnap; ++s)
+    if (t1 % g.tinv[s] == 0) g.clk[2 + s] += 1;
+  g.clk[0] = t1;
+}
+struct ClockSet {
+  int n;
+  int v[18];
+};
+__global__ void k_setclk(int* clk, ClockSet c) {
+  const int q = threadIdx.x;
+  if (q < c.n) clk[q] = c.v[q];
 }
 
 // snapshot: out[nf][npts] for the fields in the mask list
 struct SnapArgs {
   const float* __restrict__ W;
-  float* __restrict__ out;
-  long fs, sy, sz, npts;
+  float* __restrict__ out;  // member 0, record 0
+  long fs, sy, sz, npts, bstride, mstride;
+  const int* __restrict__ clk;
+  int cidx, tinv, max_records;
   int s0, s1, s2, c0, c1, t0, t1, t2;
   int nf;
   int flist[9];
 };
 __global__ void k_snapshot(SnapArgs g) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= g.npts) return;
+  if (p >= g.npts || (g.clk[0] + 1) % g.tinv != 0) return;
+  const int rec = g.clk[g.cidx];
+  if (rec >= g.max_records) return;
+  const long slot = blockIdx.y;
+  const float* W = g.W + slot * g.bstride;
+  float* out = g.out + slot * g.mstride + (long)rec * g.nf * g.npts;
   const int a = (int)(p % g.c0);
   const long r = p / g.c0;
   const int b = (int)(r % g.c1);
   const int d = (int)(r / g.c1);
   const long c = (g.s0 + a * g.t0) + g.sy * (long)(g.s1 + b * g.t1) + g.sz * (long)(g.s2 + d * g.t2);
-  for (int q = 0; q < g.nf; ++q) g.out[q * g.npts + p] = g.W[g.flist[q] * g.fs + c];
+  for (int q = 0; q < g.nf; ++q) out[q * g.npts + p] = W[g.flist[q] * g.fs + c];
 }
 
 // snapshot capture into reduced-precision storage: one workgroup per 16^3
@@ -240,19 +289,25 @@ __global__ void k_snapshot(SnapArgs g) {
 // the FP32 snapshot ([field][point], point index i fastest).
 struct PackArgs {
   const float* __restrict__ W;
-  uint16_t* __restrict__ codes;  // [nf][npts] of this record
-  float* __restrict__ scales;    // [nf][nblk] of this record
-  long fs, sy, sz, npts;
+  uint16_t* __restrict__ codes;  // member 0, record 0: [nf][npts] per record
+  float* __restrict__ scales;    // member 0, record 0: [nf][nblk] per record
+  long fs, sy, sz, npts, bstride, cstride, sstride;
+  const int* __restrict__ clk;
+  int cidx, tinv, max_records;
   int s0, s1, s2, c0, c1, c2, t0, t1, t2, nb0, nb1, nblk;
   int nf, fmt;
   int flist[9];
 };
 __global__ __launch_bounds__(256) void k_snapshot_pack(PackArgs g) {
+  if ((g.clk[0] + 1) % g.tinv != 0) return;
+  const int rr = g.clk[g.cidx];
+  if (rr >= g.max_records) return;
+  const long slot = blockIdx.z;
   const int blk = blockIdx.x, q = blockIdx.y;
   const int b0 = blk % g.nb0, r = blk / g.nb0, b1 = r % g.nb1, b2 = r / g.nb1;
   const int a = b0 * SGT_B + (int)(threadIdx.x & 15), b = b1 * SGT_B + (int)(threadIdx.x >> 4);
   const bool in_ab = a < g.c0 && b < g.c1;
-  const float* W = g.W + g.flist[q] * g.fs;
+  const float* W = g.W + slot * g.bstride + g.flist[q] * g.fs;
   float v[SGT_B];
   float m = 0.f;
 #pragma unroll
@@ -271,8 +326,8 @@ __global__ __launch_bounds__(256) void k_snapshot_pack(PackArgs g) {
   __syncthreads();
   const float amax = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
   const float inv = sgt_inv(g.fmt, amax);
-  if (threadIdx.x == 0) g.scales[(long)q * g.nblk + blk] = sgt_scale(g.fmt, amax);
-  uint16_t* out = g.codes + (long)q * g.npts;
+  if (threadIdx.x == 0) g.scales[slot * g.sstride + ((long)rr * g.nf + q) * g.nblk + blk] = sgt_scale(g.fmt, amax);
+  uint16_t* out = g.codes + slot * g.cstride + ((long)rr * g.nf + q) * g.npts;
 #pragma unroll
   for (int d = 0; d < SGT_B; ++d) {
     const int dd = b2 * SGT_B + d;
@@ -325,13 +380,15 @@ struct PmlFace {
 struct Snapshot {
   int subs[3], subc[3], subt[3], tinv, nf, flist[9];
   long npts;
-  int max_records, nrec = 0;
-  float* d = nullptr;           // FP32 records [nrec][nf][npts]
+  int max_records, nrec = 0;   // nrec: captures so far (record index of the active members)
+  std::vector<int> cnt;         // records held by each member slot
+  float* d = nullptr;           // FP32 records [slot][max_records][nf][npts]
   int fmt = SGT_FP32;           // or BF16 / FP16 codes with per-block scales
   int nb[3]{}, nblk = 0;
-  uint16_t* codes = nullptr;    // [nrec][nf][npts]
-  float* scales = nullptr;      // [nrec][nf][nblk]
+  uint16_t* codes = nullptr;    // [slot][max_records][nf][npts]
+  float* scales = nullptr;      // [slot][max_records][nf][nblk]
 };
+constexpr int MAX_SNAP = 16;    // snapshots per handle (device clock counters)
 
 struct fw_solver_s {
   fw_config cfg{};
@@ -344,8 +401,8 @@ struct fw_solver_s {
   std::string err;
   bool ready = false;
 
-  float* d_sets = nullptr;  // 4 * 9 * fs (+ off)
-  float* set[4]{};          // A B C T
+  float* d_sets = nullptr;  // nbatch * 4 * 9 * fs (+ off)
+  float* set[4]{};          // A B C T of member slot 0
   float* d_media = nullptr; // lam mu rrho
   float *lam = nullptr, *mu = nullptr, *rrho = nullptr;
   float* d_1d = nullptr;
@@ -386,6 +443,22 @@ struct fw_solver_s {
   long* d_pt = nullptr;
   float* d_rec = nullptr;
   std::vector<Snapshot> snaps;
+  // batched virtual sources: members share media, metric and PML profiles;
+  // member slots are ordered by record length (longest first), so the
+  // members still running at any step are the first nact slots
+  int nbatch = 1;
+  std::vector<int> mem_steps;         // steps each member runs, by slot
+  std::vector<int> slot_of;           // member -> slot
+  std::vector<int> fmember, mmember;  // member of each force / moment source
+  long bstride = 0;                   // floats between members' register sets
+  int width = 0;                      // members per stage launch (0: all running)
+  int sel = 0;                        // member addressed by the get/set calls
+  std::vector<int> rec_cnt;           // receiver records held by each slot
+  int* d_clk = nullptr;               // device step clock: ntime, record counters
+  int* d_srow = nullptr;              // member slot of each source row
+  // captured step graphs keyed by (ntime mod 8, running members)
+  bool use_graphs = false;
+  std::map<long, hipGraphExec_t> gexec;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -432,10 +505,11 @@ int fw_create(const fw_config* cfg, fw_handle* out) {
   if (cfg->abs_number[5] > 0) h->freenode = 0;  // mod_abs_npml.F90:182-184
   hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { delete h; return fail(nullptr, FW_ERR_HIP, "hipStreamCreate failed"); }
-  const size_t setbytes = sizeof(float) * (size_t)(9 * h->fs);
-  e = hipMalloc(&h->d_sets, 4 * setbytes + 64 * sizeof(float));
-  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc wavefield sets failed"); }
-  for (int s = 0; s < 4; ++s) h->set[s] = h->d_sets + h->off + (size_t)s * 9 * h->fs;
+  h->bstride = 4L * 9 * h->fs;  // multiple of 32 floats: every member's rows stay 128-byte aligned
+  h->mem_steps.assign(1, 0x7fffffff);
+  h->slot_of.assign(1, 0);
+  e = hipMalloc(&h->d_clk, sizeof(int) * (2 + MAX_SNAP));
+  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc clock failed"); }
   e = hipMalloc(&h->d_media, sizeof(float) * (size_t)(3 * h->fs + 64));
   if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc media failed"); }
   h->lam = h->d_media + h->off; h->mu = h->lam + h->fs; h->rrho = h->mu + h->fs;
@@ -444,8 +518,6 @@ int fw_create(const fw_config* cfg, fw_handle* out) {
   if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc 1-d failed"); }
   h->xi_x = h->d_1d; h->eta_y = h->xi_x + h->nx; h->zeta_z = h->eta_y + h->ny;
   h->rz = h->zeta_z + h->nz; h->rxsin = h->rz + h->nz; h->xcot = h->rxsin + h->nx;
-  e = hipMalloc(&h->d_msurf, sizeof(float) * 3 * h->ps);
-  if (e != hipSuccess) { fw_destroy(h); return fail(nullptr, FW_ERR_NOMEM, "hipMalloc surface failed"); }
   *out = h;
   return FW_OK;
 }
@@ -461,6 +533,8 @@ int fw_destroy(fw_handle h) {
   for (auto& s : h->snaps) { hipFree(s.d); hipFree(s.codes); hipFree(s.scales); }
   for (auto& e : h->ev) hipEventDestroy(e);
   hipFree(h->d_items);
+  hipFree(h->d_clk);
+  for (auto& kv : h->gexec) hipGraphExecDestroy(kv.second);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
   return FW_OK;
@@ -637,8 +711,8 @@ int fw_set_receivers(fw_handle h, int n, int tinv, const int* indx, int max_reco
       lin[p] = (indx[3 * p] - 1) + h->sy * (long)(indx[3 * p + 1] - 1) + h->sz * (long)(indx[3 * p + 2] - 1);
     HIPCHK(hipMalloc(&h->d_pt, sizeof(long) * n));
     HIPCHK(hipMemcpy(h->d_pt, lin.data(), sizeof(long) * n, hipMemcpyHostToDevice));
-    if (max_records > 0) HIPCHK(hipMalloc(&h->d_rec, sizeof(float) * 9 * (size_t)n * max_records));
   }
+  h->ready = false;  // record storage is sized by fw_init
   return FW_OK;
 }
 
@@ -664,14 +738,9 @@ static int add_snapshot(fw_handle h, const int* subs, const int* subc, const int
   s.nblk = s.nb[0] * s.nb[1] * s.nb[2];
   s.max_records = max_records;
   s.fmt = fmt;
-  const size_t nv = (size_t)s.nf * s.npts * max_records;
-  if (fmt == SGT_FP32) {
-    HIPCHK(hipMalloc(&s.d, sizeof(float) * nv));
-  } else {
-    HIPCHK(hipMalloc(&s.codes, sizeof(uint16_t) * nv));
-    HIPCHK(hipMalloc(&s.scales, sizeof(float) * (size_t)s.nf * s.nblk * max_records));
-  }
-  h->snaps.push_back(s);
+  if ((int)h->snaps.size() >= MAX_SNAP) return fail(h, FW_ERR_ARG, std::string(who) + ": at most 16 snapshots");
+  h->snaps.push_back(s);  // storage is sized by fw_init
+  h->ready = false;
   *snap_id = (int)h->snaps.size() - 1;
   return FW_OK;
 }
@@ -788,17 +857,20 @@ static int build_sources(fw_handle h) {
       for (int q = 0; q < 343; ++q) d[q] = d[q] / s2;
     }
   };
-  std::map<long, std::vector<SrcContrib>> cells;
-  std::vector<int> smap((size_t)h->ps, -1);
+  // rows keyed by (member slot, cell); one surface plane map per slot
+  std::map<std::pair<int, long>, std::vector<SrcContrib>> cells;
+  std::vector<int> smap((size_t)h->ps * h->nbatch, -1);
   std::vector<SurfEntry> sent;
-  std::map<long, int> surf_of;  // plane index -> entry
+  std::map<long, int> surf_of;  // slot * ps + plane index -> entry
+  auto fslot = [&](int n) { return h->nbatch > 1 ? h->slot_of[h->fmember[n]] : 0; };
+  auto mslot = [&](int n) { return h->nbatch > 1 ? h->slot_of[h->mmember[n]] : 0; };
   float d[343];
   for (int n = 0; n < h->nforce; ++n) {
     const int si = h->findx[3 * n] - 1, sj = h->findx[3 * n + 1] - 1, sk = h->findx[3 * n + 2] - 1;
     // surface traction entries (src_surface), last (n, m) wins
     if (h->freenode && si >= h->ni1 && si <= h->ni2 && sj >= h->nj1 && sj <= h->nj2 && sk == h->nk2) {
       const float S = (x[si + 1] - x[si - 1]) * (y[sj + 1] - y[sj - 1]) / 4.0f * (z[sk] * z[sk]) * h->hxsin[si];
-      const long c2 = si + h->sy * (long)sj;
+      const long c2 = si + h->sy * (long)sj + h->ps * (long)fslot(n);
       auto it = surf_of.find(c2);
       SurfEntry e{n, h->fntw - 1, 0.f, S};
       if (it == surf_of.end()) {
@@ -818,7 +890,7 @@ static int build_sources(fw_handle h) {
           const long c = i + h->sy * (long)j + h->sz * (long)k;
           const float V = volume(i, j, k);
           const float w = 1.0f / (V * h->hrho[i + (size_t)h->nx * (j + (size_t)h->ny * k)]) * d[(Li + 3) + 7 * (Lj + 3) + 49 * (Lk + 3)];
-          cells[c].push_back(SrcContrib{0, n, 0, w});
+          cells[{fslot(n), c}].push_back(SrcContrib{0, n, 0, w});
         }
   }
   for (int n = 0; n < h->nmom; ++n) {
@@ -831,21 +903,23 @@ static int build_sources(fw_handle h) {
           if (i < h->ni1 || i > h->ni2 || j < h->nj1 || j > h->nj2 || k < h->nk1 || k > h->nk2) continue;
           const long c = i + h->sy * (long)j + h->sz * (long)k;
           const float V = volume(i, j, k);
-          cells[c].push_back(SrcContrib{1, n, 0, d[(Li + 3) + 7 * (Lj + 3) + 49 * (Lk + 3)] / V});
+          cells[{mslot(n), c}].push_back(SrcContrib{1, n, 0, d[(Li + 3) + 7 * (Lj + 3) + 49 * (Lk + 3)] / V});
         }
   }
   for (auto& kv : surf_of) {
     const SurfEntry& e = sent[kv.second];
     smap[kv.first] = kv.second;
-    const long c = kv.first + h->sz * (long)h->nk2;
-    cells[c].push_back(SrcContrib{2, e.n, e.m, 1.0f / e.S});
+    const int slot = (int)(kv.first / h->ps);
+    const long c = kv.first % h->ps + h->sz * (long)h->nk2;
+    cells[{slot, c}].push_back(SrcContrib{2, e.n, e.m, 1.0f / e.S});
   }
   // device tables
   std::vector<long> cell;
-  std::vector<int> ptr{0};
+  std::vector<int> ptr{0}, rowslot;
   std::vector<SrcContrib> con;
   for (auto& kv : cells) {
-    cell.push_back(kv.first);
+    rowslot.push_back(kv.first.first);
+    cell.push_back(kv.first.second);
     for (auto& s : kv.second) con.push_back(s);
     ptr.push_back((int)con.size());
   }
@@ -853,6 +927,7 @@ static int build_sources(fw_handle h) {
   h->n_surf = (int)sent.size();
   int bad = 0;
   bad |= dalloc((void**)&h->d_scell, sizeof(long) * cell.size(), cell.data());
+  bad |= dalloc((void**)&h->d_srow, sizeof(int) * rowslot.size(), rowslot.data());
   bad |= dalloc((void**)&h->d_sptr, sizeof(int) * ptr.size(), ptr.data());
   bad |= dalloc((void**)&h->d_scon, sizeof(SrcContrib) * con.size(), con.data());
   bad |= dalloc((void**)&h->d_smap, sizeof(int) * smap.size(), smap.data());
@@ -869,12 +944,13 @@ static int build_sources(fw_handle h) {
 }
 
 static int zero_state(fw_handle h) {
-  const long n = 4L * 9 * h->fs;
-  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->set[0] - h->off + 0, n + h->off, 0.f);
-  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((3 * h->ps + 255) / 256)), dim3(256), 0, h->stream, h->d_msurf, 3 * h->ps, 0.f);
+  const long n = (long)h->nbatch * h->bstride + h->off;
+  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_sets, n, 0.f);
+  const long ns = 3L * h->ps * h->nbatch;
+  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, h->stream, h->d_msurf, ns, 0.f);
   for (auto& f : h->face)
     if (f.on) {
-      const long m = 4L * 6 * f.cnt;
+      const long m = 4L * 6 * f.cnt * h->nbatch;
       hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, h->stream, f.d, m, 0.f);
     }
   HIPCHK(hipGetLastError());
@@ -886,9 +962,73 @@ static int zero_state(fw_handle h) {
 
 static int build_items(fw_handle h);
 
+// Device storage sized by the member count: register sets, surface copies,
+// receiver and snapshot records; record counters reset.
+static int alloc_state(fw_handle h) {
+  const int B = h->nbatch;
+  hipFree(h->d_sets); hipFree(h->d_msurf); hipFree(h->d_rec);
+  h->d_sets = nullptr; h->d_msurf = nullptr; h->d_rec = nullptr;
+  HIPCHK(hipMalloc(&h->d_sets, sizeof(float) * ((size_t)B * h->bstride + 64)));
+  for (int q = 0; q < 4; ++q) h->set[q] = h->d_sets + h->off + (size_t)q * 9 * h->fs;
+  HIPCHK(hipMalloc(&h->d_msurf, sizeof(float) * 3 * (size_t)h->ps * B));
+  if (h->npt > 0 && h->max_rec > 0) HIPCHK(hipMalloc(&h->d_rec, sizeof(float) * 9 * (size_t)h->npt * h->max_rec * B));
+  h->nrec = 0;
+  h->rec_cnt.assign(B, 0);
+  for (auto& sn : h->snaps) {
+    hipFree(sn.d); hipFree(sn.codes); hipFree(sn.scales);
+    sn.d = nullptr; sn.codes = nullptr; sn.scales = nullptr;
+    const size_t nv = (size_t)sn.nf * sn.npts * sn.max_records * B;
+    if (sn.fmt == SGT_FP32) {
+      HIPCHK(hipMalloc(&sn.d, sizeof(float) * nv));
+    } else {
+      HIPCHK(hipMalloc(&sn.codes, sizeof(uint16_t) * nv));
+      HIPCHK(hipMalloc(&sn.scales, sizeof(float) * (size_t)sn.nf * sn.nblk * sn.max_records * B));
+    }
+    sn.nrec = 0;
+    sn.cnt.assign(B, 0);
+  }
+  for (auto& kv : h->gexec) hipGraphExecDestroy(kv.second);
+  h->gexec.clear();
+  return FW_OK;
+}
+
+int fw_set_batch(fw_handle h, int nbatch, const int* force_member, const int* moment_member, const int* nsteps) {
+  if (!h || nbatch < 1 || nbatch > 4095 || !nsteps || (h->nforce > 0 && !force_member) ||
+      (h->nmom > 0 && !moment_member))
+    return fail(h, FW_ERR_ARG, "fw_set_batch: bad argument (1 <= nbatch <= 4095)");
+  std::vector<int> fm(force_member, force_member + h->nforce), mm(moment_member, moment_member + h->nmom);
+  for (int v : fm)
+    if (v < 0 || v >= nbatch) return fail(h, FW_ERR_ARG, "fw_set_batch: force member out of range");
+  for (int v : mm)
+    if (v < 0 || v >= nbatch) return fail(h, FW_ERR_ARG, "fw_set_batch: moment member out of range");
+  for (int b = 0; b < nbatch; ++b)
+    if (nsteps[b] < 0) return fail(h, FW_ERR_ARG, "fw_set_batch: negative record length");
+  std::vector<int> order(nbatch);
+  for (int b = 0; b < nbatch; ++b) order[b] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nsteps[a] > nsteps[b]; });
+  h->nbatch = nbatch;
+  h->slot_of.assign(nbatch, 0);
+  h->mem_steps.assign(nbatch, 0);
+  for (int q = 0; q < nbatch; ++q) {
+    h->slot_of[order[q]] = q;
+    h->mem_steps[q] = nsteps[order[q]];
+  }
+  h->fmember = fm;
+  h->mmember = mm;
+  h->sel = 0;
+  h->ready = false;
+  return FW_OK;
+}
+
 int fw_init(fw_handle h) {
   if (!h) return FW_ERR_ARG;
   if (!h->have_grid || !h->have_media) return fail(h, FW_ERR_STATE, "fw_init: grid and media must be set first");
+  if (h->nbatch > 1 && ((int)h->fmember.size() != h->nforce || (int)h->mmember.size() != h->nmom))
+    return fail(h, FW_ERR_STATE, "fw_init: sources changed after fw_set_batch; call fw_set_batch again");
+  {
+    int rc = alloc_state(h);
+    if (rc) return rc;
+  }
   build_pml(h);
   // device PML profiles: per axis b1 = 1/B, ad = A + D/B, dd = D/B
   hipFree(h->d_pml);
@@ -918,7 +1058,7 @@ int fw_init(fw_handle h) {
     if (n % 2 == 0) f.hi[ax] = f.lo[ax] + N; else f.lo[ax] = f.hi[ax] - N;
     f.cnt = 1;
     for (int d = 0; d < 3; ++d) { f.ex[d] = f.hi[d] - f.lo[d] + 1; f.cnt *= f.ex[d]; }
-    HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt));
+    HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt * h->nbatch));
   }
   int rc = build_items(h);
   if (rc) return rc;
@@ -1023,7 +1163,7 @@ static int build_items(fw_handle h) {
 
 extern "C" {
 
-static int launch_stage(fw_handle h, int ntime, int st) {
+static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   const int* bp = BLOCK_PAT[((ntime % 8) + 8) % 8];
   const int flip = st & 1;
   const int xf = flip ? !bp[0] : bp[0], yf = flip ? !bp[1] : bp[1], zf = flip ? !bp[2] : bp[2];
@@ -1038,7 +1178,6 @@ static int launch_stage(fw_handle h, int ntime, int st) {
   static const float RKA[4] = {0.5f, 0.5f, 1.0f, 0.0f};
   const float RKB[4] = {1.0f / 6.0f, 1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 6.0f};
   const float tinc = st == 0 ? 0.f : RKA[st - 1];
-  const float ts = ((float)ntime + tinc) * h->cfg.stept;
   const float a = RKA[st] * h->cfg.stept, b = RKB[st] * h->cfg.stept;
 
   if (h->freenode) {
@@ -1046,9 +1185,10 @@ static int launch_stage(fw_handle h, int ntime, int st) {
     s.win = win; s.msurf = h->d_msurf; s.smap = h->d_smap; s.ent = h->d_sent;
     s.ft0 = h->d_ft0; s.ftime = h->d_ftime; s.ffreq = h->d_ffreq; s.F = h->d_F;
     s.fs = h->fs; s.sy = h->sy; s.sz = h->sz; s.ps = h->ps; s.fblk = (long)h->fntw * h->nforce;
+    s.bstride = h->bstride; s.clk = h->d_clk;
     s.ni1 = h->ni1; s.ni2 = h->ni2; s.nj1 = h->nj1; s.nj2 = h->nj2; s.nk2 = h->nk2;
-    s.ntw = h->fntw; s.stf_id = h->fstf; s.ntime = ntime; s.save = st == 0; s.ts = ts; s.stept = h->cfg.stept;
-    dim3 blk(64, 4), grd((h->cfg.ni + 63) / 64, (h->cfg.nj + 3) / 4);
+    s.ntw = h->fntw; s.stf_id = h->fstf; s.save = st == 0; s.tinc = tinc; s.stept = h->cfg.stept;
+    dim3 blk(64, 4), grd((h->cfg.ni + 63) / 64, (h->cfg.nj + 3) / 4, nact);
     hipLaunchKernelGGL(k_surface, grd, blk, 0, h->stream, s);
   }
   StageArgs g{};
@@ -1080,9 +1220,13 @@ static int launch_stage(fw_handle h, int ntime, int st) {
   g.items = h->d_items;
   g.nitems = h->n_pml + h->n_lean;
   g.npml = h->n_pml;
-  if (g.nitems > 0) {
-    hipLaunchKernelGGL(vel_table(tc, pat, kind), dim3(g.nitems), sblk, 0, h->stream, g);
-    hipLaunchKernelGGL(str_table(tc, pat, kind), dim3(g.nitems), sblk, 0, h->stream, g);
+  g.bstride = h->bstride;
+  const int W = h->width > 0 ? h->width : nact;
+  for (int s0 = 0; s0 < nact && g.nitems > 0; s0 += W) {  // members s0 .. s0+W-1 in one launch
+    g.slot0 = s0;
+    const dim3 grid(g.nitems, std::min(W, nact - s0));
+    hipLaunchKernelGGL(vel_table(tc, pat, kind), grid, sblk, 0, h->stream, g);
+    hipLaunchKernelGGL(str_table(tc, pat, kind), grid, sblk, 0, h->stream, g);
   }
   if (h->timing) {
     hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
@@ -1097,61 +1241,132 @@ static int launch_stage(fw_handle h, int ntime, int st) {
     s.fblk = (long)h->fntw * h->nforce; s.fntw = h->fntw; s.fstf = h->fstf;
     s.mt0 = h->d_mt0; s.mtime = h->d_mtime; s.mfreq = h->d_mfreq; s.M = h->d_M;
     s.mblk = (long)h->mntw * h->nmom; s.mntw = h->mntw; s.mstf = h->mstf;
-    s.fs = h->fs; s.ncell = h->n_srccell; s.ntime = ntime; s.kind = kind;
-    s.ts = ts; s.stept = h->cfg.stept; s.a = a; s.b = b;
+    s.fs = h->fs; s.bstride = h->bstride; s.rowslot = h->d_srow; s.clk = h->d_clk;
+    s.ncell = h->n_srccell; s.nact = nact; s.kind = kind;
+    s.tinc = tinc; s.stept = h->cfg.stept; s.a = a; s.b = b;
     hipLaunchKernelGGL(k_src, dim3((h->n_srccell + 127) / 128), dim3(128), 0, h->stream, s);
   }
   return FW_OK;
 }
 
-static int capture_outputs(fw_handle h, int t1) {
+// Host bookkeeping of the captures after step ntime (the device kernels
+// find their record slots on the device clock): storage checks, then the
+// record counts of the running members.
+static int account_outputs(fw_handle h, int ntime, int nact) {
+  const int t1 = ntime + 1;
   if (h->npt > 0 && h->d_rec && t1 % h->pt_tinv == 0) {
     if (h->nrec >= h->max_rec) return fail(h, FW_ERR_STATE, "receiver storage full (max_records)");
-    hipLaunchKernelGGL(k_receivers, dim3((h->npt * 9 + 255) / 256), dim3(256), 0, h->stream, h->set[0], h->fs, h->d_pt,
-                       h->npt, h->d_rec + (size_t)h->nrec * 9 * h->npt);
     h->nrec++;
+    for (int q = 0; q < nact; ++q) h->rec_cnt[q] = h->nrec;
   }
-  for (auto& s : h->snaps) {
-    if (t1 % s.tinv != 0) continue;
-    if (s.nrec >= s.max_records) return fail(h, FW_ERR_STATE, "snapshot storage full (max_records)");
-    if (s.fmt != SGT_FP32) {
-      PackArgs a{};
-      a.W = h->set[0];
-      a.codes = s.codes + (size_t)s.nrec * s.nf * s.npts;
-      a.scales = s.scales + (size_t)s.nrec * s.nf * s.nblk;
-      a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = s.npts;
-      a.s0 = s.subs[0]; a.s1 = s.subs[1]; a.s2 = s.subs[2];
-      a.c0 = s.subc[0]; a.c1 = s.subc[1]; a.c2 = s.subc[2];
-      a.t0 = s.subt[0]; a.t1 = s.subt[1]; a.t2 = s.subt[2];
-      a.nb0 = s.nb[0]; a.nb1 = s.nb[1]; a.nblk = s.nblk; a.nf = s.nf; a.fmt = s.fmt;
-      for (int q = 0; q < s.nf; ++q) a.flist[q] = s.flist[q];
-      hipLaunchKernelGGL(k_snapshot_pack, dim3((unsigned)s.nblk, (unsigned)s.nf), dim3(256), 0, h->stream, a);
-      s.nrec++;
-      continue;
-    }
-    SnapArgs a{};
-    a.W = h->set[0]; a.out = s.d + (size_t)s.nrec * s.nf * s.npts;
-    a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = s.npts;
-    a.s0 = s.subs[0]; a.s1 = s.subs[1]; a.s2 = s.subs[2]; a.c0 = s.subc[0]; a.c1 = s.subc[1];
-    a.t0 = s.subt[0]; a.t1 = s.subt[1]; a.t2 = s.subt[2]; a.nf = s.nf;
-    for (int q = 0; q < s.nf; ++q) a.flist[q] = s.flist[q];
-    hipLaunchKernelGGL(k_snapshot, dim3((unsigned)((s.npts + 255) / 256)), dim3(256), 0, h->stream, a);
-    s.nrec++;
+  for (auto& sn : h->snaps) {
+    if (t1 % sn.tinv != 0) continue;
+    if (sn.nrec >= sn.max_records) return fail(h, FW_ERR_STATE, "snapshot storage full (max_records)");
+    sn.nrec++;
+    for (int q = 0; q < nact; ++q) sn.cnt[q] = sn.nrec;
   }
   return FW_OK;
 }
 
-int fw_step(fw_handle h, int ntime) {
-  if (!h) return FW_ERR_ARG;
-  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_step: call fw_init first");
-  if (ntime < 0) return fail(h, FW_ERR_ARG, "fw_step: ntime must be >= 0");
+// Output kernels after step ntime (all of them when recording a graph: each
+// decides on the device clock whether this step is a capture step), then
+// the clock tick.
+static void enqueue_outputs(fw_handle h, int ntime, int nact, bool all) {
+  const int t1 = ntime + 1;
+  if (h->npt > 0 && h->d_rec && (all || t1 % h->pt_tinv == 0))
+    hipLaunchKernelGGL(k_receivers, dim3((h->npt * 9 + 255) / 256, nact), dim3(256), 0, h->stream, h->set[0], h->fs,
+                       h->bstride, h->d_pt, h->npt, h->d_rec, h->max_rec, h->pt_tinv, h->d_clk);
+  for (size_t q = 0; q < h->snaps.size(); ++q) {
+    const Snapshot& sn = h->snaps[q];
+    if (!all && t1 % sn.tinv != 0) continue;
+    if (sn.fmt != SGT_FP32) {
+      PackArgs a{};
+      a.W = h->set[0]; a.codes = sn.codes; a.scales = sn.scales;
+      a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = sn.npts; a.bstride = h->bstride;
+      a.cstride = (long)sn.max_records * sn.nf * sn.npts; a.sstride = (long)sn.max_records * sn.nf * sn.nblk;
+      a.clk = h->d_clk; a.cidx = 2 + (int)q; a.tinv = sn.tinv; a.max_records = sn.max_records;
+      a.s0 = sn.subs[0]; a.s1 = sn.subs[1]; a.s2 = sn.subs[2];
+      a.c0 = sn.subc[0]; a.c1 = sn.subc[1]; a.c2 = sn.subc[2];
+      a.t0 = sn.subt[0]; a.t1 = sn.subt[1]; a.t2 = sn.subt[2];
+      a.nb0 = sn.nb[0]; a.nb1 = sn.nb[1]; a.nblk = sn.nblk; a.nf = sn.nf; a.fmt = sn.fmt;
+      for (int f = 0; f < sn.nf; ++f) a.flist[f] = sn.flist[f];
+      hipLaunchKernelGGL(k_snapshot_pack, dim3((unsigned)sn.nblk, (unsigned)sn.nf, (unsigned)nact), dim3(256), 0,
+                         h->stream, a);
+      continue;
+    }
+    SnapArgs a{};
+    a.W = h->set[0]; a.out = sn.d;
+    a.fs = h->fs; a.sy = h->sy; a.sz = h->sz; a.npts = sn.npts; a.bstride = h->bstride;
+    a.mstride = (long)sn.max_records * sn.nf * sn.npts;
+    a.clk = h->d_clk; a.cidx = 2 + (int)q; a.tinv = sn.tinv; a.max_records = sn.max_records;
+    a.s0 = sn.subs[0]; a.s1 = sn.subs[1]; a.s2 = sn.subs[2]; a.c0 = sn.subc[0]; a.c1 = sn.subc[1];
+    a.t0 = sn.subt[0]; a.t1 = sn.subt[1]; a.t2 = sn.subt[2]; a.nf = sn.nf;
+    for (int f = 0; f < sn.nf; ++f) a.flist[f] = sn.flist[f];
+    hipLaunchKernelGGL(k_snapshot, dim3((unsigned)((sn.npts + 255) / 256), (unsigned)nact), dim3(256), 0, h->stream, a);
+  }
+  TickArgs tk{};
+  tk.clk = h->d_clk;
+  tk.rec_tinv = (h->npt > 0 && h->d_rec) ? h->pt_tinv : 0;
+  tk.nsnap = (int)h->snaps.size();
+  for (size_t q = 0; q < h->snaps.size(); ++q) tk.tinv[q] = h->snaps[q].tinv;
+  hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, h->stream, tk);
+}
+
+// members still running at step ntime (slots are ordered longest first)
+static int running(fw_handle h, int ntime) {
+  int n = 0;
+  while (n < h->nbatch && h->mem_steps[n] > ntime) ++n;
+  return n;
+}
+
+static int enqueue_step(fw_handle h, int ntime, int nact, bool all) {
   for (int st = 0; st < 4; ++st) {
-    int rc = launch_stage(h, ntime, st);
+    int rc = launch_stage(h, ntime, st, nact);
     if (rc) return rc;
   }
-  HIPCHK(hipGetLastError());
-  return capture_outputs(h, ntime + 1);
+  enqueue_outputs(h, ntime, nact, all);
+  return FW_OK;
 }
+
+// The step as a graph: launches depend on ntime only through its block
+// position (the F/B pattern); times and record slots come from the clock.
+static int step_graph(fw_handle h, int ntime, int nact, hipGraphExec_t* out) {
+  const long key = (long)(ntime % 8) * 4096 + nact;
+  auto it = h->gexec.find(key);
+  if (it != h->gexec.end()) {
+    *out = it->second;
+    return FW_OK;
+  }
+  HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_step(h, ntime, nact, true);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(h, FW_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(h, FW_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  h->gexec[key] = ex;
+  *out = ex;
+  return FW_OK;
+}
+
+static int set_clock(fw_handle h, int ntime) {
+  ClockSet c{};
+  c.n = 2 + (int)h->snaps.size();
+  c.v[0] = ntime;
+  c.v[1] = h->nrec;
+  for (size_t q = 0; q < h->snaps.size(); ++q) c.v[2 + q] = h->snaps[q].nrec;
+  hipLaunchKernelGGL(k_setclk, dim3(1), dim3(64), 0, h->stream, h->d_clk, c);
+  HIPCHK(hipGetLastError());
+  return FW_OK;
+}
+
+int fw_step(fw_handle h, int ntime) { return fw_run(h, ntime, 1); }
 
 int fw_enable_timing(fw_handle h, int on) {
   if (!h) return FW_ERR_ARG;
@@ -1161,7 +1376,8 @@ int fw_enable_timing(fw_handle h, int on) {
 
 int fw_run(fw_handle h, int ntime0, int nsteps) {
   if (!h) return FW_ERR_ARG;
-  if (nsteps < 0) return fail(h, FW_ERR_ARG, "fw_run: nsteps must be >= 0");
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_run: call fw_init first");
+  if (nsteps < 0 || ntime0 < 0) return fail(h, FW_ERR_ARG, "fw_run: ntime0 and nsteps must be >= 0");
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (h->timing) {
     const size_t need = 2 * (size_t)nsteps * 4;
@@ -1175,10 +1391,22 @@ int fw_run(fw_handle h, int ntime0, int nsteps) {
     HIPCHK(hipEventCreate(&t1));
     hipEventRecord(t0, h->stream);
   }
+  int rc = set_clock(h, ntime0);
+  if (rc) return rc;
   for (int n = 0; n < nsteps; ++n) {
-    int rc = fw_step(h, ntime0 + n);
-    if (rc) return rc;
+    const int ntime = ntime0 + n;
+    const int nact = running(h, ntime);
+    if (nact == 0) break;  // every member has run its record length
+    if ((rc = account_outputs(h, ntime, nact))) return rc;
+    if (h->use_graphs && !h->timing) {
+      hipGraphExec_t ex = nullptr;
+      if ((rc = step_graph(h, ntime, nact, &ex))) return rc;
+      HIPCHK(hipGraphLaunch(ex, h->stream));
+    } else {
+      if ((rc = enqueue_step(h, ntime, nact, false))) return rc;
+    }
   }
+  HIPCHK(hipGetLastError());
   if (h->timing) {
     hipEventRecord(t1, h->stream);
     HIPCHK(hipEventSynchronize(t1));
@@ -1212,76 +1440,163 @@ int fw_sync(fw_handle h) {
   return FW_OK;
 }
 
+// slot of the member the get/set calls address
+static int sel_slot(fw_handle h) { return h->nbatch > 1 ? h->slot_of[h->sel] : 0; }
+
 int fw_get_receivers(fw_handle h, float* out, int* nrecords) {
   if (!h) return FW_ERR_ARG;
   HIPCHK(hipStreamSynchronize(h->stream));
-  if (nrecords) *nrecords = h->nrec;
-  if (out && h->nrec > 0 && h->npt > 0)
-    HIPCHK(hipMemcpy(out, h->d_rec, sizeof(float) * 9 * (size_t)h->npt * h->nrec, hipMemcpyDeviceToHost));
+  const int sl = sel_slot(h);
+  const int n = h->rec_cnt.empty() ? 0 : h->rec_cnt[sl];
+  if (nrecords) *nrecords = n;
+  if (out && n > 0 && h->npt > 0)
+    HIPCHK(hipMemcpy(out, h->d_rec + (size_t)sl * h->max_rec * h->npt * 9, sizeof(float) * 9 * (size_t)h->npt * n,
+                     hipMemcpyDeviceToHost));
+  return FW_OK;
+}
+
+static int snap_check(fw_handle h, int id, bool packed, const char* who) {
+  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, std::string(who) + ": bad id");
+  if (!h->ready) return fail(h, FW_ERR_STATE, std::string(who) + ": call fw_init first");
+  if (packed && h->snaps[id].fmt == SGT_FP32) return fail(h, FW_ERR_ARG, std::string(who) + ": FP32 snapshot");
+  if (!packed && h->snaps[id].fmt != SGT_FP32) return fail(h, FW_ERR_ARG, std::string(who) + ": packed snapshot");
+  HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
 
 int fw_get_snapshot(fw_handle h, int id, float* out, int* nrecords) {
-  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_get_snapshot: bad id");
-  if (h->snaps[id].fmt != SGT_FP32) return fail(h, FW_ERR_ARG, "fw_get_snapshot: packed snapshot, use fw_get_snapshot_packed");
-  HIPCHK(hipStreamSynchronize(h->stream));
+  if (int rc = snap_check(h, id, false, "fw_get_snapshot")) return rc;
   const Snapshot& s = h->snaps[id];
-  if (nrecords) *nrecords = s.nrec;
-  if (out && s.nrec > 0)
-    HIPCHK(hipMemcpy(out, s.d, sizeof(float) * (size_t)s.nf * s.npts * s.nrec, hipMemcpyDeviceToHost));
+  const int sl = sel_slot(h), n = s.cnt[sl];
+  if (nrecords) *nrecords = n;
+  if (out && n > 0)
+    HIPCHK(hipMemcpy(out, s.d + (size_t)sl * s.max_records * s.nf * s.npts, sizeof(float) * (size_t)s.nf * s.npts * n,
+                     hipMemcpyDeviceToHost));
   return FW_OK;
 }
 
 int fw_get_snapshot_packed(fw_handle h, int id, uint16_t* codes, float* scales, int* nrecords, long* nblocks) {
-  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_get_snapshot_packed: bad id");
+  if (int rc = snap_check(h, id, true, "fw_get_snapshot_packed")) return rc;
   const Snapshot& s = h->snaps[id];
-  if (s.fmt == SGT_FP32) return fail(h, FW_ERR_ARG, "fw_get_snapshot_packed: FP32 snapshot, use fw_get_snapshot");
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (nrecords) *nrecords = s.nrec;
+  const int sl = sel_slot(h), n = s.cnt[sl];
+  if (nrecords) *nrecords = n;
   if (nblocks) *nblocks = s.nblk;
-  if (codes && s.nrec > 0)
-    HIPCHK(hipMemcpy(codes, s.codes, sizeof(uint16_t) * (size_t)s.nf * s.npts * s.nrec, hipMemcpyDeviceToHost));
-  if (scales && s.nrec > 0)
-    HIPCHK(hipMemcpy(scales, s.scales, sizeof(float) * (size_t)s.nf * s.nblk * s.nrec, hipMemcpyDeviceToHost));
+  if (codes && n > 0)
+    HIPCHK(hipMemcpy(codes, s.codes + (size_t)sl * s.max_records * s.nf * s.npts,
+                     sizeof(uint16_t) * (size_t)s.nf * s.npts * n, hipMemcpyDeviceToHost));
+  if (scales && n > 0)
+    HIPCHK(hipMemcpy(scales, s.scales + (size_t)sl * s.max_records * s.nf * s.nblk,
+                     sizeof(float) * (size_t)s.nf * s.nblk * n, hipMemcpyDeviceToHost));
   return FW_OK;
 }
 
 int fw_snapshot_packed_device(fw_handle h, int id, uint16_t** codes, float** scales, int* nrecords, long* npoints,
                               long* nblocks) {
-  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_snapshot_packed_device: bad id");
+  if (int rc = snap_check(h, id, true, "fw_snapshot_packed_device")) return rc;
   const Snapshot& s = h->snaps[id];
-  if (s.fmt == SGT_FP32) return fail(h, FW_ERR_ARG, "fw_snapshot_packed_device: FP32 snapshot, use fw_snapshot_device");
-  HIPCHK(hipStreamSynchronize(h->stream));
-  if (codes) *codes = s.codes;
-  if (scales) *scales = s.scales;
-  if (nrecords) *nrecords = s.nrec;
+  const int sl = sel_slot(h);
+  if (codes) *codes = s.codes + (size_t)sl * s.max_records * s.nf * s.npts;
+  if (scales) *scales = s.scales + (size_t)sl * s.max_records * s.nf * s.nblk;
+  if (nrecords) *nrecords = s.cnt[sl];
   if (npoints) *npoints = s.npts;
   if (nblocks) *nblocks = s.nblk;
   return FW_OK;
 }
 
 int fw_snapshot_device(fw_handle h, int id, float** dptr, int* nrecords, long* npoints) {
-  if (!h || id < 0 || id >= (int)h->snaps.size()) return fail(h, FW_ERR_ARG, "fw_snapshot_device: bad id");
-  if (h->snaps[id].fmt != SGT_FP32) return fail(h, FW_ERR_ARG, "fw_snapshot_device: packed snapshot, use fw_snapshot_packed_device");
-  HIPCHK(hipStreamSynchronize(h->stream));
+  if (int rc = snap_check(h, id, false, "fw_snapshot_device")) return rc;
   const Snapshot& s = h->snaps[id];
-  if (dptr) *dptr = s.d;
-  if (nrecords) *nrecords = s.nrec;
+  const int sl = sel_slot(h);
+  if (dptr) *dptr = s.d + (size_t)sl * s.max_records * s.nf * s.npts;
+  if (nrecords) *nrecords = s.cnt[sl];
   if (npoints) *npoints = s.npts;
   return FW_OK;
 }
 
 int fw_get_wavefield(fw_handle h, int f, float* out) {
   if (!h || f < 0 || f >= 9 || !out) return fail(h, FW_ERR_ARG, "fw_get_wavefield: bad argument");
-  return download_volume(h, out, h->set[0] + (size_t)f * h->fs);
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_get_wavefield: call fw_init first");
+  return download_volume(h, out, h->set[0] + (size_t)sel_slot(h) * h->bstride + (size_t)f * h->fs);
 }
 
 int fw_set_wavefield(fw_handle h, int f, const float* in) {
   if (!h || f < 0 || f >= 9 || !in) return fail(h, FW_ERR_ARG, "fw_set_wavefield: bad argument");
   if (!h->ready) return fail(h, FW_ERR_STATE, "fw_set_wavefield: call fw_init first");
-  int rc = upload_volume(h, h->set[0] + (size_t)f * h->fs, in);
+  int rc = upload_volume(h, h->set[0] + (size_t)sel_slot(h) * h->bstride + (size_t)f * h->fs, in);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+int fw_select_member(fw_handle h, int member) {
+  if (!h || member < 0 || member >= h->nbatch) return fail(h, FW_ERR_ARG, "fw_select_member: member out of range");
+  h->sel = member;
+  return FW_OK;
+}
+
+int fw_use_graphs(fw_handle h, int on) {
+  if (!h) return FW_ERR_ARG;
+  h->use_graphs = on != 0;
+  return FW_OK;
+}
+
+int fw_set_batch_width(fw_handle h, int width) {
+  if (!h || width < 0) return fail(h, FW_ERR_ARG, "fw_set_batch_width: width must be >= 0");
+  h->width = width;
+  for (auto& kv : h->gexec) hipGraphExecDestroy(kv.second);
+  h->gexec.clear();
+  return FW_OK;
+}
+
+static void reset_records(fw_handle h) {
+  h->nrec = 0;
+  std::fill(h->rec_cnt.begin(), h->rec_cnt.end(), 0);
+  for (auto& sn : h->snaps) {
+    sn.nrec = 0;
+    std::fill(sn.cnt.begin(), sn.cnt.end(), 0);
+  }
+}
+
+int fw_autotune(fw_handle h, int trial_steps, int* tile, int* width, float* ms_per_step) {
+  if (!h || trial_steps < 1) return fail(h, FW_ERR_ARG, "fw_autotune: trial_steps must be >= 1");
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_autotune: call fw_init first");
+  const bool timing = h->timing;
+  h->timing = false;
+  std::vector<int> widths{0};
+  for (int w = 1; w < h->nbatch; w *= 2) widths.push_back(w);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 3.4e38f;
+  int bt = h->tilecfg, bw = h->width, rc = FW_OK;
+  for (int tc = 0; tc < NTILECFG && !rc; ++tc) {
+    h->tilecfg = tc;
+    if ((rc = build_items(h))) break;
+    for (int w : widths) {
+      if ((rc = fw_set_batch_width(h, w)) || (rc = zero_state(h))) break;
+      reset_records(h);
+      if ((rc = fw_run(h, 0, 1))) break;  // warm-up (graph capture, first touch)
+      hipEventRecord(e0, h->stream);
+      if ((rc = fw_run(h, 1, trial_steps))) break;
+      hipEventRecord(e1, h->stream);
+      hipEventSynchronize(e1);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) { best = ms; bt = tc; bw = w; }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  h->timing = timing;
+  h->tilecfg = bt;
+  if (!rc) rc = build_items(h);
+  if (!rc) rc = fw_set_batch_width(h, bw);
+  if (!rc) rc = zero_state(h);
+  reset_records(h);
+  if (rc) return rc;
+  if (tile) *tile = bt;
+  if (width) *width = bw;
+  if (ms_per_step) *ms_per_step = best / trial_steps;
   return FW_OK;
 }
 

@@ -84,7 +84,14 @@ struct StageArgs {
   const float* __restrict__ pdd;
   int n1, setin, setout, pml;
   FaceArgs face[6];
+  // batched virtual sources: member slot0 + blockIdx.y; its register sets
+  // start bstride floats after the previous member's, its surface copy
+  // 3*ps after, its PML auxiliaries 24*cnt after (per face)
+  long bstride;
+  int slot0;
 };
+
+__device__ __forceinline__ int member_slot(const StageArgs& g) { return g.slot0 + (int)blockIdx.y; }
 
 __device__ __forceinline__ long face_index(const FaceArgs& f, int i, int j, int k) {
   return (long)(i - f.lo0) + (long)f.ex0 * ((j - f.lo1) + (long)f.ex1 * (k - f.lo2));
@@ -93,21 +100,25 @@ __device__ __forceinline__ bool in_face(const FaceArgs& f, int i, int j, int k) 
   return f.on && i >= f.lo0 && i <= f.hi0 && j >= f.lo1 && j <= f.hi1 && k >= f.lo2 && k <= f.hi2;
 }
 // auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105)
+__device__ __forceinline__ float* aux_base(const FaceArgs& f, const StageArgs& g) {
+  return f.d + (long)member_slot(g) * 24 * f.cnt;
+}
 template <int KIND>
 __device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, long li, int v, float psi, float dpsi) {
   const long S = 6 * f.cnt, q = v * f.cnt + li;
+  float* const d = aux_base(f, g);
   if constexpr (KIND == 0) {
-    f.d[g.setout * S + q] = psi + g.a * dpsi;
-    f.d[3 * S + q] = psi + g.b * dpsi;
+    d[g.setout * S + q] = psi + g.a * dpsi;
+    d[3 * S + q] = psi + g.b * dpsi;
   } else if constexpr (KIND == 1) {
-    f.d[g.setout * S + q] = f.d[q] + g.a * dpsi;
-    f.d[3 * S + q] = f.d[3 * S + q] + g.b * dpsi;
+    d[g.setout * S + q] = d[q] + g.a * dpsi;
+    d[3 * S + q] = d[3 * S + q] + g.b * dpsi;
   } else {
-    f.d[g.setout * S + q] = f.d[3 * S + q] + g.b * dpsi;
+    d[g.setout * S + q] = d[3 * S + q] + g.b * dpsi;
   }
 }
 __device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, long li, int v) {
-  return f.d[g.setin * 6 * f.cnt + v * f.cnt + li];
+  return aux_base(f, g)[g.setin * 6 * f.cnt + v * f.cnt + li];
 }
 
 // CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
@@ -325,7 +336,11 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
                         (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
-  const float* __restrict__ W = g.win;
+  const long bo = (long)member_slot(g) * g.bstride;
+  const float* __restrict__ W = g.win + bo;
+  float* __restrict__ wout = g.wout + bo;
+  const float* __restrict__ mm = g.m + bo;
+  float* __restrict__ tt = g.t + bo;
   const float* __restrict__ pTxx = W + TXX * fs;
   const float* __restrict__ pTyy = W + TYY * fs;
   const float* __restrict__ pTxy = W + TXY * fs;
@@ -405,8 +420,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         const long e = f * fs + c;
-        mo[q][f] = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : g.m + e) : 0.f;
-        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(g.t + e) : 0.f;
+        mo[q][f] = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : mm + e) : 0.f;
+        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(tt + e) : 0.f;
       }
     }
     // prefetch plane k+1
@@ -464,10 +479,10 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
           if constexpr (KIND == 2) {
-            st_nt(g.wout + e, to[q][f] + g.b * h[f]);
+            st_nt(wout + e, to[q][f] + g.b * h[f]);
           } else {
-            st_nt(g.wout + e, mo[q][f] + g.a * h[f]);
-            st_nt(g.t + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
+            st_nt(wout + e, mo[q][f] + g.a * h[f]);
+            st_nt(tt + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
           }
         }
       }
@@ -509,7 +524,11 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
                         (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
-  const float* __restrict__ W = g.win;
+  const long bo = (long)member_slot(g) * g.bstride;
+  const float* __restrict__ W = g.win + bo;
+  float* __restrict__ wout = g.wout + bo;
+  const float* __restrict__ mm = g.m + bo;
+  float* __restrict__ tt = g.t + bo;
   const float* __restrict__ pV0 = W + VX * fs;
   const float* __restrict__ pV1 = W + VY * fs;
   const float* __restrict__ pV2 = W + VZ * fs;
@@ -574,11 +593,11 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
         const long e = (TXX + f) * fs + c;
-        float mv = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : g.m + e) : 0.f;
+        float mv = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : mm + e) : 0.f;
         if (surf && comp[q] && KIND != 2 && (f == 2 || f == 4 || f == 5))
-          mv = g.msurf[(f == 2 ? 0 : (f == 4 ? 1 : 2)) * g.ps + cc[q]];
+          mv = g.msurf[(long)member_slot(g) * 3 * g.ps + (f == 2 ? 0 : (f == 4 ? 1 : 2)) * g.ps + cc[q]];
         mo[q][f] = mv;
-        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(g.t + e) : 0.f;
+        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(tt + e) : 0.f;
       }
     }
     // prefetch plane k+1
@@ -646,10 +665,10 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
         for (int f = 0; f < 6; ++f) {
           const long e = (TXX + f) * fs + c;
           if constexpr (KIND == 2) {
-            st_nt(g.wout + e, to[q][f] + g.b * h[f]);
+            st_nt(wout + e, to[q][f] + g.b * h[f]);
           } else {
-            st_nt(g.wout + e, mo[q][f] + g.a * h[f]);
-            st_nt(g.t + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
+            st_nt(wout + e, mo[q][f] + g.a * h[f]);
+            st_nt(tt + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
           }
         }
       }

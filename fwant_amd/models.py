@@ -84,6 +84,38 @@ class Model:
         return 8 * ((self.nt + 1 + 7) // 8)
 
 
+@dataclass
+class Batch:
+    """Members of a batched solve: the member of each force / moment source
+    and the number of steps each member runs."""
+    force_member: np.ndarray
+    moment_member: np.ndarray
+    nsteps: np.ndarray
+
+
+def virtual_sources(model, stations, nsteps, fz=1.0e16, f0=None):
+    """Ambient-noise virtual sources: one vertical surface force per station
+    (C indices (i, j) on the free surface), member b running nsteps[b] steps.
+    Returns a copy of `model` whose force table holds every station's force
+    (the model's own sources are dropped) and the matching Batch."""
+    import copy
+    m = copy.copy(model)
+    f0 = model.force.stf_freq[0] if (f0 is None and model.force.n) else (f0 or 0.5)
+    n = len(stations)
+    s = Sources(n=n, ntw=1, stf_id=STF_RICKER)
+    s.indx = np.array([[i + 1, j + 1, m.nk2 + 1] for i, j in stations], np.int32)
+    s.shift = np.zeros((n, 3), np.float32)
+    s.t0 = np.zeros(n, np.float32)
+    s.stf_time = np.array([1.2 / f0], np.float32)
+    s.stf_freq = np.array([f0], np.float32)
+    s.comps = np.zeros((3, n), np.float32)
+    s.comps[2] = fz
+    m.force = s
+    m.moment = Sources(comps=np.zeros((6, 0), np.float32))
+    b = Batch(np.arange(n, dtype=np.int32), np.zeros(0, np.int32), np.asarray(nsteps, np.int32))
+    return m, b
+
+
 def spherical_grid(ni, nj, nk, h, colat0=np.pi / 2, lon0=0.0, r_top=R_EARTH):
     """Uniform spherical grid: dtheta = dphi = h / r_top, dr = h; the top
     interior plane (k = nk2) sits at r_top.  Returns ghosted float64 x, y, z."""

@@ -19,13 +19,15 @@ from ._lib import FwConfig, check, fptr, iptr, dptr
 
 
 class Solver:
-    def __init__(self, model, metric=None, max_records=None, snapshots=()):
+    def __init__(self, model, metric=None, max_records=None, snapshots=(), batch=None):
         """model: fwant_amd.models.Model.  metric: None -> build on the GPU in
         FP64 (fw_build_metric); or a dict with xsin, xcot, xi_x, eta_y, zeta_z
         (float32) as read from metric_mpi*.nc.  snapshots: iterable of dicts
         with subs, subc, subt (1-based), tinv, fields (names), max_records and
         optionally format ("fp32", "bf16" or "fp16": 16-bit codes with a
-        per-16^3-block FP32 scale)."""
+        per-16^3-block FP32 scale).  batch: None, or a Batch (member of each
+        force / moment source of the model and the steps each member runs):
+        the members advance together in the same launches."""
         L = _lib.lib()
         self.L = L
         self.model = model
@@ -107,6 +109,13 @@ class Solver:
             nblk = int(np.prod([(c + 15) // 16 for c in s["subc"]]))
             self.snaps.append(dict(s, id=sid.value, nf=len(s["fields"]), npts=int(np.prod(subc)), nblk=nblk,
                                    format=fmt))
+        self.nbatch = 1
+        if batch is not None:
+            fm = np.ascontiguousarray(batch.force_member, np.int32)
+            mm = np.ascontiguousarray(batch.moment_member, np.int32)
+            ns = np.ascontiguousarray(batch.nsteps, np.int32)
+            check(L.fw_set_batch(h, len(ns), iptr(fm), iptr(mm), iptr(ns)), h, "fw_set_batch")
+            self.nbatch = len(ns)
         check(L.fw_init(h), h, "fw_init")
         self.ntime = 0
 
@@ -115,6 +124,23 @@ class Solver:
         t0 = self.ntime if ntime0 is None else ntime0
         check(self.L.fw_run(self.h, int(t0), int(nsteps)), self.h, "fw_run")
         self.ntime = t0 + nsteps
+
+    def select(self, member):
+        """Address member `member` of a batch in the output / wavefield calls."""
+        check(self.L.fw_select_member(self.h, int(member)), self.h, "fw_select_member")
+
+    def use_graphs(self, on=True):
+        check(self.L.fw_use_graphs(self.h, 1 if on else 0), self.h, "fw_use_graphs")
+
+    def set_width(self, width):
+        check(self.L.fw_set_batch_width(self.h, int(width)), self.h, "fw_set_batch_width")
+
+    def autotune(self, trial_steps=4):
+        """Pick the fastest tile shape and batch width; returns (tile, width, ms per step)."""
+        t, w, ms = C.c_int(), C.c_int(), C.c_float()
+        check(self.L.fw_autotune(self.h, int(trial_steps), C.byref(t), C.byref(w), C.byref(ms)), self.h,
+              "fw_autotune")
+        return t.value, w.value, ms.value
 
     def sync(self):
         check(self.L.fw_sync(self.h), self.h, "fw_sync")

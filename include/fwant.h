@@ -154,6 +154,29 @@ int fw_snapshot_packed_device(fw_handle h, int snap_id, uint16_t **codes, float 
 int fw_get_wavefield(fw_handle h, int field, float *out);
 int fw_set_wavefield(fw_handle h, int field, const float *in);
 
+/* Batched virtual sources (no counterpart in the reference, which runs one
+ * solver job per virtual source).  nbatch members share the grid, media,
+ * metric, PML profiles, receivers and snapshot definitions; each has its own
+ * wavefield, PML auxiliaries and output records.  Force source n belongs to
+ * member force_member[n], moment source n to moment_member[n]; member b runs
+ * nsteps[b] steps from ntime0 = 0 (uneven record lengths) and then stays
+ * frozen.  Call after fw_set_force / fw_set_moment and before fw_init.  All
+ * running members advance in the same stage launches (fw_set_batch_width
+ * members per launch; 0 = all).  The get/set calls above address the member
+ * chosen with fw_select_member (default 0).  Storage scales with nbatch. */
+int fw_set_batch(fw_handle h, int nbatch, const int *force_member, const int *moment_member, const int *nsteps);
+int fw_select_member(fw_handle h, int member);
+int fw_set_batch_width(fw_handle h, int width);
+/* Run each step as a captured HIP graph (one per F/B block position and
+ * number of running members, built on first use); source times and record
+ * slots come from a device-side step clock, so replays need no host
+ * arguments.  Results are identical to the launch-by-launch path. */
+int fw_use_graphs(fw_handle h, int on);
+/* Time trial_steps steps for every stage-kernel tile shape and batch width
+ * (powers of two below nbatch, and all), keep the fastest, and restore the
+ * zero initial state with empty records.  Call after fw_init. */
+int fw_autotune(fw_handle h, int trial_steps, int *tile, int *width, float *ms_per_step);
+
 /* Per-kernel timing of the last fw_run (milliseconds of the main stage
  * kernels summed, measured with HIP events on the handle's stream). */
 int fw_enable_timing(fw_handle h, int on);
