@@ -19,6 +19,11 @@ Prints ONE JSON line:
              profiles/stage_traffic_gradient_400x400x200.json)
   cpu_baseline  the FP32 C oracle (OpenMP) on the host cores, timed on a
              bounded sample of the same workload (2 full steps)
+  pipeline   station-pair kernels/hour, plain (one FP32 solve per station)
+             vs batched + BF16 SGT (one batched solve with step graphs and
+             the autotuner), on a 256x256x128 sample: 4 virtual sources of
+             240/200/160/120 steps, SGT every 10 steps on a 2x decimated
+             grid, 4 pairs; with the kernels' relative L2 difference
 """
 import argparse
 import json
@@ -104,6 +109,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def pipeline_sample():
+    import numpy as np
+    from scipy import signal
+    from fwant_amd import models, pipeline
+    m = models.gradient_400x400x200(ni=256, nj=256, nk=128, h=500.0, seed=4)
+    # 4 stations around the centre, 6-12 km apart, so that both SGTs of a
+    # pair overlap within the records
+    c = 3 + 128
+    st = [(c - 10, c - 8), (c + 12, c - 6), (c - 4, c + 14), (c + 8, c + 10)]
+    nsteps = [240, 200, 160, 120]
+    pairs = [(0, 1), (0, 2), (1, 3), (2, 3)]
+    tinv = 10
+    knt = max(nsteps) // tinv
+    t = np.arange(1, knt + 1) * m.stept * tinv
+    V = np.sin(2 * np.pi * 0.5 * t).astype(np.float32)
+    U = np.cos(2 * np.pi * 0.5 * t).astype(np.float32)
+    b, a = signal.butter(2, 0.4)
+    kw = dict(tinv=tinv, step=2, V=V, U=U, win=(3, 11), M0=1e16, filt=("P2", b, a))
+    kp, tp = pipeline.plain(m, st, nsteps, pairs, **kw)
+    kb, tb = pipeline.batched(m, st, nsteps, pairs, fmt="bf16", **kw)
+    errs = [float(np.linalg.norm(y.astype(np.float64) - x) / np.linalg.norm(x))
+            for P, B in zip(kp, kb) for x, y in zip(P, B) if np.linalg.norm(x) > 0]
+    tot_p = tp["setup"] + tp["steps"] + tp["kernel"]
+    tot_b = tb["setup"] + tb["steps"] + tb["kernel"]
+    plain_h, batch_h = len(pairs) / tot_p * 3600.0, len(pairs) / tot_b * 3600.0
+    log(f"pipeline: plain {tp}, batched bf16 {tb}, kernel rel-L2 max {max(errs):.2e} over {len(errs)} kernels")
+    return {"workload": "256x256x128 gradient model, 4 virtual sources (240/200/160/120 steps), "
+                        "6-component stress SGT every 10 steps on a 2x decimated grid, 4 station pairs",
+            "unit": "kernels/hour",
+            "plain": dict(value=plain_h, **tp),
+            "batched_bf16": dict(value=batch_h, **tb),
+            "speedup": batch_h / plain_h,
+            "speedup_steps_only": tp["steps"] / tb["steps"],
+            "kernel_rel_l2_bf16_vs_fp32": max(errs), "kernels_compared": len(errs)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +155,7 @@ def main():
     ap.add_argument("--nk", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--traffic", type=str, default=TRAFFIC_JSON,
                     help="JSON file with measured HBM bytes per RK stage (written by --summarize-pmc)")
     ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "OUT"),
@@ -197,6 +239,8 @@ def main():
         log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M cell-updates/s on {cores} threads")
         o.close()
 
+    pipe = None if args.no_pipeline else pipeline_sample()
+
     out = {
         "metric": "cell-updates/s + % HBM peak; batched+compressed vs plain at ≤1% kernel err",
         "value": value,
@@ -219,6 +263,7 @@ def main():
                      "kernel": "RK stage = k_vel + k_str (operator sweep fused with the RK update)",
                      "mean_launch_us": avg_ms * 1e3},
         "cpu_baseline": cpu,
+        "pipeline": pipe,
     }
     print(json.dumps(out), flush=True)
 

@@ -157,13 +157,26 @@ class DeviceArray:
     """A float32/float64 array in HBM owned by this object (allocated through
     the library's HIP runtime)."""
 
-    def __init__(self, shape, dtype=np.float32):
+    def __init__(self, shape, dtype=np.float32, _ptr=None, strides=None):
         self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
-        p = C.c_void_p()
-        kcheck(lib().fw_device_alloc(self.nbytes, C.byref(p)), "fw_device_alloc")
-        self.ptr = p.value
+        # element strides of a (component, sample, point) series; None: C order
+        self.strides = strides
+        self.owner = _ptr is None
+        if _ptr is None:
+            p = C.c_void_p()
+            kcheck(lib().fw_device_alloc(self.nbytes, C.byref(p)), "fw_device_alloc")
+            self.ptr = p.value
+        else:
+            self.ptr = int(_ptr)
+
+    @classmethod
+    def wrap(cls, ptr, shape, dtype=np.float32, strides=None):
+        """A view of device memory owned elsewhere (e.g. snapshot storage):
+        `strides` (component, sample) element strides of a 3-D series whose
+        points are contiguous; the view covers prod(shape) elements."""
+        return cls(shape, dtype, _ptr=ptr, strides=strides)
 
     @classmethod
     def from_host(cls, a):
@@ -179,6 +192,7 @@ class DeviceArray:
 
     def copy(self):
         d = DeviceArray(self.shape, self.dtype)
+        d.strides = self.strides
         kcheck(lib().fw_memcpy_d2d(C.c_void_p(d.ptr), C.c_void_p(self.ptr), self.nbytes), "fw_memcpy_d2d")
         return d
 
@@ -186,7 +200,7 @@ class DeviceArray:
         return int(np.prod(self.shape))
 
     def free(self):
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and getattr(self, "owner", False):
             lib().fw_device_free(C.c_void_p(self.ptr))
             self.ptr = None
 

@@ -30,9 +30,16 @@ def _ptr(t):
 
 
 def _series(t):
-    """(ncomp, knt, npt) tensor -> base pointer, component stride, time stride."""
+    """(ncomp, knt, npt) series -> base pointer, component stride, time stride."""
     nc, knt, npt = t.shape
-    return _ptr(t), knt * npt, npt, knt, npt
+    cs, ts = t.strides if t.strides is not None else (knt * npt, npt)
+    return _ptr(t), cs, ts, knt, npt
+
+
+def snapshot_series(ptr, nrec, nf, npts):
+    """The records of a snapshot ([record][field][point] storage) as a
+    (field, sample, point) series view, without copying."""
+    return DeviceArray.wrap(ptr, (nf, nrec, npts), np.float32, strides=(npts, nf * npts))
 
 
 def _stream():
