@@ -75,7 +75,7 @@ struct StageArgs {
   int nx, ny;
   int ni1, ni2, nj1, nj2, nk1, nk2;
   int freenode;
-  const int4* __restrict__ items;  // work items: (i0, j0, k0, k1) of a tile over a chunk of planes
+  const int4* __restrict__ items;  // work items (i0 | ncol << 16, j0 | nrow << 16, k0, k1): a tile over a chunk of planes
   int nitems, npml;
   float a, b;
   // CFS-PML: per-axis profiles 1/B, A + D/B, D/B (x: nx, y: ny, z: nz entries)
@@ -275,22 +275,23 @@ __device__ __forceinline__ float zc(const float* w) {
 
 // Halo slot of one thread in one LDS plane: x halo (XL left + XR right
 // columns of the TY rows, threads tid < 4*TY) or y halo (YL + YR rows of the
-// TX columns, every thread).  Out-of-volume slots (partial edge tiles) hold
-// zero and never feed a computed cell.
+// TX columns, every thread).  Slots outside the volume or beyond what the
+// tile's computed columns ie / rows je read (partial tiles) hold zero and
+// never feed a computed cell.
 struct HaloSlot {
   unsigned off;
   int lds;
   bool ok;
 };
 template <int XF, int TY, int LW>
-__device__ __forceinline__ HaloSlot xhalo_slot(const StageArgs& g, int i0, int j0, int tid) {
+__device__ __forceinline__ HaloSlot xhalo_slot(const StageArgs& g, int i0, int j0, int ie, int je, int tid) {
   constexpr int XL = XF ? 1 : 3;
   HaloSlot h{0u, 0, false};
   if (tid < 4 * TY) {
     const int r = tid % TY, s = tid / TY;
     const int col = s < XL ? i0 - XL + s : i0 + TX + (s - XL);
     const int row = j0 + r;
-    h.ok = col < g.nx && row < g.ny;
+    h.ok = col < g.nx && col <= ie + 3 && row < g.ny && row <= je;
     h.off = h.ok ? (unsigned)(col + g.sy * row) : 0u;
     h.lds = (r + 3) * LW + (col - i0 + 3);
   } else {
@@ -299,14 +300,14 @@ __device__ __forceinline__ HaloSlot xhalo_slot(const StageArgs& g, int i0, int j
   return h;
 }
 template <int YF, int TY, int LW>
-__device__ __forceinline__ HaloSlot yhalo_slot(const StageArgs& g, int i0, int j0, int tid) {
+__device__ __forceinline__ HaloSlot yhalo_slot(const StageArgs& g, int i0, int j0, int ie, int je, int tid) {
   constexpr int YL = YF ? 1 : 3;
   HaloSlot h{0u, -1, false};
   if (tid >= 4 * TX) return h;
   const int c = tid % TX, s = tid / TX;
   const int row = s < YL ? j0 - YL + s : j0 + TY + (s - YL);
   const int col = i0 + c;
-  h.ok = col < g.nx && row < g.ny;
+  h.ok = col < g.nx && col <= ie && row < g.ny && row <= je + 3;
   h.off = h.ok ? (unsigned)(col + g.sy * row) : 0u;
   h.lds = (row - j0 + 3) * LW + (c + 3);
   return h;
@@ -331,9 +332,11 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
   constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, NL = 5;
   float(*sh)[NL * PL] = reinterpret_cast<float(*)[NL * PL]>(shbase);  // Txx Txy Txz Tyy Tyz, double buffered
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-  const int i0 = it.x, j0 = it.y, k0 = it.z, k1 = it.w;
-  const bool tile_pml = PML && (g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && i0 + TX - 1 >= g.face[1].lo0) ||
-                        (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
+  const int i0 = it.x & 0xffff, j0 = it.y & 0xffff, k0 = it.z, k1 = it.w;
+  // last computed column and row: tiles end at CFS-PML slab boundaries
+  const int ie = i0 + (it.x >> 16) - 1, je = j0 + (it.y >> 16) - 1;
+  const bool tile_pml = PML && ((g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && ie >= g.face[1].lo0) ||
+                                (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && je >= g.face[3].lo1));
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
   const long bo = (long)member_slot(g) * g.bstride;
@@ -348,11 +351,11 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
   const float* __restrict__ pTxz = W + TXZ * fs;
   const float* __restrict__ pTyz = W + TYZ * fs;
   const int i = i0 + tx;
-  const bool iok = i < g.nx, icomp = i <= g.ni2;
+  const bool iok = i < g.nx && i <= ie + 3, icomp = i <= ie;
   const int ic = min(i, g.nx - 1);
   const float xi = g.xi_x[ic], rxs = g.rxsin[ic], xc = g.xcot[ic];
-  const HaloSlot hx = xhalo_slot<XF, TY, LW>(g, i0, j0, tid);
-  const HaloSlot hy = yhalo_slot<YF, TY, LW>(g, i0, j0, tid);
+  const HaloSlot hx = xhalo_slot<XF, TY, LW>(g, i0, j0, ie, je, tid);
+  const HaloSlot hy = yhalo_slot<YF, TY, LW>(g, i0, j0, ie, je, tid);
 
   float w[RY][3][5];  // Tzz Txz Tyz
   float cTxx[RY], cTyy[RY], cTxy[RY];
@@ -362,8 +365,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
     const int j = j0 + ty + BYT * q;
-    ok[q] = iok && j < g.ny;
-    comp[q] = icomp && j <= g.nj2;
+    ok[q] = iok && j < g.ny && j <= je + 3;
+    comp[q] = icomp && j <= je;
     cc[q] = ok[q] ? (unsigned)(i + g.sy * j) : 0u;
     et[q] = g.eta_y[min(j, g.ny - 1)];
     fluid[q] = comp[q] ? g.fluidcol[cc[q]] != 0 : false;
@@ -519,9 +522,11 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH;
   float(*sh)[3 * PL] = reinterpret_cast<float(*)[3 * PL]>(shbase);  // Vx Vy Vz, double buffered
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
-  const int i0 = it.x, j0 = it.y, k0 = it.z, k1 = it.w;
-  const bool tile_pml = PML && (g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && i0 + TX - 1 >= g.face[1].lo0) ||
-                        (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && j0 + TY - 1 >= g.face[3].lo1);
+  const int i0 = it.x & 0xffff, j0 = it.y & 0xffff, k0 = it.z, k1 = it.w;
+  // last computed column and row: tiles end at CFS-PML slab boundaries
+  const int ie = i0 + (it.x >> 16) - 1, je = j0 + (it.y >> 16) - 1;
+  const bool tile_pml = PML && ((g.face[0].on && i0 <= g.face[0].hi0) || (g.face[1].on && ie >= g.face[1].lo0) ||
+                                (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && je >= g.face[3].lo1));
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
   const long bo = (long)member_slot(g) * g.bstride;
@@ -533,11 +538,11 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   const float* __restrict__ pV1 = W + VY * fs;
   const float* __restrict__ pV2 = W + VZ * fs;
   const int i = i0 + tx;
-  const bool iok = i < g.nx, icomp = i <= g.ni2;
+  const bool iok = i < g.nx && i <= ie + 3, icomp = i <= ie;
   const int ic = min(i, g.nx - 1);
   const float xi = g.xi_x[ic], rxs = g.rxsin[ic], xc = g.xcot[ic];
-  const HaloSlot hx = xhalo_slot<XF, TY, LW>(g, i0, j0, tid);
-  const HaloSlot hy = yhalo_slot<YF, TY, LW>(g, i0, j0, tid);
+  const HaloSlot hx = xhalo_slot<XF, TY, LW>(g, i0, j0, ie, je, tid);
+  const HaloSlot hy = yhalo_slot<YF, TY, LW>(g, i0, j0, ie, je, tid);
 
   float w[RY][3][5];
   bool ok[RY], comp[RY];
@@ -546,8 +551,8 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
     const int j = j0 + ty + BYT * q;
-    ok[q] = iok && j < g.ny;
-    comp[q] = icomp && j <= g.nj2;
+    ok[q] = iok && j < g.ny && j <= je + 3;
+    comp[q] = icomp && j <= je;
     cc[q] = ok[q] ? (unsigned)(i + g.sy * j) : 0u;
     et[q] = g.eta_y[min(j, g.ny - 1)];
     const unsigned c0 = cc[q] + sz * (unsigned)k0;
