@@ -236,3 +236,25 @@ class Solver:
 
     def __exit__(self, *a):
         self.close()
+
+
+def build_metric(model):
+    """The FP32 metric of `model`'s grid built on the GPU in FP64 (as
+    seis3d_metric writes it to metric_mpi*.nc)."""
+    m = models_lite_copy(model)
+    s = Solver(m)
+    met = s.metric
+    s.close()
+    return met
+
+
+def models_lite_copy(model):
+    """A copy of `model` without sources and receivers (for set-up only work)."""
+    import copy
+    from . import models
+    m = copy.copy(model)
+    m.force = models.Sources()
+    m.moment = models.Sources(comps=np.zeros((6, 0), np.float32))
+    m.rec_indx = np.zeros((0, 3), np.int32)
+    return m
+
