@@ -24,7 +24,8 @@ SYMBOLS = (
     "fw_get_snapshot_packed", "fw_snapshot_packed_device", "fwk_sgt_decode", "fw_set_batch",
     "fw_select_member", "fw_set_batch_width", "fw_use_graphs", "fw_autotune", "fw_get_wavefield",
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
-    "fwk_stress2strain", "fwk_scatter", "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
+    "fwk_stress2strain", "fwk_scatter", "fwk_scatter_param", "fwk_vscatter", "fwk_eoshift", "fwk_reparam",
+    "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
     "fwk_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
     "fw_memcpy_d2d", "fw_device_sync",
 )
@@ -93,6 +94,12 @@ def _sig(lib):
         "fwk_stress2strain": (C.c_int, [vp, C.c_long, C.c_long, C.c_long, C.c_int, vp, vp, vp]),
         "fwk_scatter": (C.c_int, [vp, C.c_long, C.c_long, vp, C.c_long, C.c_long, C.c_long, C.c_int,
                                   C.c_float, vp, vp, vp, vp, vp]),
+        "fwk_scatter_param": (C.c_int, [C.c_int, vp, C.c_long, C.c_long, vp, C.c_long, C.c_long, C.c_long, C.c_int,
+                                        C.c_float, vp, vp, vp, vp, vp]),
+        "fwk_vscatter": (C.c_int, [vp, C.c_long, C.c_long, vp, C.c_long, C.c_long, C.c_long, C.c_int, C.c_float,
+                                   vp, vp, vp]),
+        "fwk_eoshift": (C.c_int, [vp, C.c_long, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, vp]),
+        "fwk_reparam": (C.c_int, [C.c_int, C.c_long, vp, vp, vp, vp, vp, vp, vp]),
         "fwk_filter": (C.c_int, [C.c_int, C.c_long, C.c_int, C.c_int, dp, dp, vp, vp, vp]),
         "fwk_scatter2kernel": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, fp, fp,
                                          vp, vp, vp, vp, vp, vp, vp]),

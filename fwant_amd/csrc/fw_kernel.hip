@@ -66,7 +66,10 @@ __global__ void k_sgt_decode(int fmt, const uint16_t* __restrict__ codes, const 
 // Each thread owns one point and NB consecutive outputs n0..n0+NB-1 (1-based);
 // it walks m = 1 .. n0+NB-2 keeping a sliding register window of the source
 // strain S(n-m) for its NB outputs, so R(m) is loaded once per m.
+// PARAM selects the output pair: 0 alpha/beta (2(l+2u) x1, 4u (x2-x1)),
+// 1 lambda/mu (l x1, 2u x2; SI_ker_sta_lamu.F90:1333-1336).
 constexpr int NB = 8;
+template <int PARAM>
 __global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ R, long rcs, long rts,
                                                  const float* __restrict__ S, long scs, long sts, long npt, int knt,
                                                  float dt, const float* __restrict__ lam, const float* __restrict__ mu,
@@ -118,9 +121,106 @@ __global__ __launch_bounds__(256) void k_scatter(const float* __restrict__ R, lo
     const int n = n0 + q;
     if (n > knt) continue;
     const double x1 = a1[q] * (double)dt, x2 = a2[q] * (double)dt;
-    outA[(long)(n - 1) * npt + p] = (float)(x1 * 2.0 * (l + 2.0 * u));
-    outB[(long)(n - 1) * npt + p] = (float)(2.0 * (x2 - x1) * 2.0 * u);
+    if constexpr (PARAM == 0) {
+      outA[(long)(n - 1) * npt + p] = (float)(x1 * 2.0 * (l + 2.0 * u));
+      outB[(long)(n - 1) * npt + p] = (float)(2.0 * (x2 - x1) * 2.0 * u);
+    } else {
+      outA[(long)(n - 1) * npt + p] = (float)(x1 * l);
+      outB[(long)(n - 1) * npt + p] = (float)(2.0 * x2 * u);
+    }
   }
+}
+
+// vel2scatter3 for one receiver component (SI_ker_sta_rho.F90:1478-1552):
+// x1(n) = dt sum_{m<n} (VxR+VyR+VzR)(m) (VxS+VyS+VzS)(n-m), output x1*rho.
+// Same sliding-window structure as k_scatter on the component sums.
+__global__ __launch_bounds__(256) void k_vscatter(const float* __restrict__ R, long rcs, long rts,
+                                                  const float* __restrict__ S, long scs, long sts, long npt, int knt,
+                                                  float dt, const float* __restrict__ rho, float* __restrict__ out) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  const int nblk = (knt + NB - 1) / NB;
+  const int n0 = 1 + (nblk - 1 - (int)blockIdx.y) * NB;
+  float sv[NB];
+  double a1[NB];
+  auto ssum = [&](int idx) {
+    if (idx < 1 || idx > knt) return 0.f;
+    const float* q = S + (long)(idx - 1) * sts + p;
+    return q[0] + q[scs] + q[2 * scs];
+  };
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    a1[q] = 0.0;
+    sv[q] = ssum(n0 + q - 1);
+  }
+  const int mmax = min(n0 + NB - 2, knt - 1);
+  for (int m = 1; m <= mmax; ++m) {
+    const float* r = R + (long)(m - 1) * rts + p;
+    const float rv = r[0] + r[rcs] + r[2 * rcs];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) a1[q] += (double)(rv * sv[q]);
+#pragma unroll
+    for (int q = NB - 1; q > 0; --q) sv[q] = sv[q - 1];
+    sv[0] = ssum(n0 - m - 1);
+  }
+  const double rh = rho[p];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int n = n0 + q;
+    if (n > knt) continue;
+    out[(long)(n - 1) * npt + p] = (float)(a1[q] * (double)dt * rh);
+  }
+}
+
+// eoshift along time of ncomp series (Fortran semantics: shift > 0 moves
+// sample n+shift to n, the vacated end is zero; shift < 0 the reverse),
+// as SI_ker_EQ.F90 applies it to the receiver SGT.  In place: a thread owns
+// one point and walks n in the direction that reads ahead of its writes.
+__global__ void k_eoshift(float* __restrict__ E, long cs, long ts, long npt, int knt, int ncomp, int shift) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  for (int c = 0; c < ncomp; ++c) {
+    float* e = E + c * cs + p;
+    if (shift >= 0) {
+      for (int n = 0; n < knt; ++n) e[(long)n * ts] = n + shift < knt ? e[(long)(n + shift) * ts] : 0.f;
+    } else {
+      for (int n = knt - 1; n >= 0; --n) e[(long)n * ts] = n + shift >= 0 ? e[(long)(n + shift) * ts] : 0.f;
+    }
+  }
+}
+
+// kernel re-parameterisation, in place on (a, b) phase and amplitude kernels:
+// mode 0 lambda/mu -> kappa/mu (SI_ker_lamu2kamu.F90:823-830), 1 lambda/mu ->
+// Vp/Vs (SI_ker_lamu2vpvs.F90:823-830), 2 Vp/Vs -> kappa/mu
+// (SI_ker_vpvs2kamu.F90:823-831); FP32 as the reference.
+__device__ __forceinline__ void reparam1(int mode, float l, float u, float& a, float& b) {
+  if (mode == 0) {
+    a = a / l;
+    b = b / u / 2.0f;
+    b = (2.0f * b - 2.0f * a / 3.0f) * u;
+    a = a * (l + 2.0f * u / 3.0f);
+  } else if (mode == 1) {
+    a = a / l;
+    b = b / u / 2.0f;
+    b = 2.0f * (b - a) * 2.0f * u;
+    a = a * 2.0f * (l + 2.0f * u);
+  } else {
+    a = a / (2.0f * (l + 2.0f * u));
+    b = b / u / 4.0f + a;
+    b = (2.0f * b - 2.0f * a / 3.0f) * u;
+    a = a * (l + 2.0f * u / 3.0f);
+  }
+}
+__global__ void k_reparam(int mode, long npt, const float* __restrict__ lam, const float* __restrict__ mu,
+                          float* __restrict__ Kap, float* __restrict__ Kaq, float* __restrict__ Kbp,
+                          float* __restrict__ Kbq) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npt) return;
+  const float l = lam[p], u = mu[p];
+  float ap = Kap[p], bp = Kbp[p], aq = Kaq[p], bq = Kbq[p];
+  reparam1(mode, l, u, ap, bp);
+  reparam1(mode, l, u, aq, bq);
+  Kap[p] = ap; Kbp[p] = bp; Kaq[p] = aq; Kbq[p] = bq;
 }
 
 // IIR filter of SI_ker_sta.F90:1547-1557 over the padded series of
@@ -289,9 +389,48 @@ int fwk_scatter(const float* R, long rcs, long rts, const float* S, long scs, lo
                 const float* lam, const float* mu, float* outA, float* outB, void* stream) {
   if (!R || !S || !lam || !mu || !outA || !outB || npt <= 0 || knt <= 0)
     return kfail(FW_ERR_ARG, "fwk_scatter: bad argument");
+  return fwk_scatter_param(0, R, rcs, rts, S, scs, sts, npt, knt, dt, lam, mu, outA, outB, stream);
+}
+
+int fwk_scatter_param(int param, const float* R, long rcs, long rts, const float* S, long scs, long sts, long npt,
+                      int knt, float dt, const float* lam, const float* mu, float* outA, float* outB, void* stream) {
+  if (!R || !S || !lam || !mu || !outA || !outB || npt <= 0 || knt <= 0 || (param != 0 && param != 1))
+    return kfail(FW_ERR_ARG, "fwk_scatter_param: bad argument");
   const unsigned gy = (unsigned)((knt + NB - 1) / NB);
-  hipLaunchKernelGGL(k_scatter, dim3(nblocks(npt, 256), gy), dim3(256), 0, (hipStream_t)stream, R, rcs, rts, S, scs,
-                     sts, npt, knt, dt, lam, mu, outA, outB);
+  if (param == 0)
+    hipLaunchKernelGGL(k_scatter<0>, dim3(nblocks(npt, 256), gy), dim3(256), 0, (hipStream_t)stream, R, rcs, rts, S,
+                       scs, sts, npt, knt, dt, lam, mu, outA, outB);
+  else
+    hipLaunchKernelGGL(k_scatter<1>, dim3(nblocks(npt, 256), gy), dim3(256), 0, (hipStream_t)stream, R, rcs, rts, S,
+                       scs, sts, npt, knt, dt, lam, mu, outA, outB);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
+
+int fwk_vscatter(const float* R, long rcs, long rts, const float* S, long scs, long sts, long npt, int knt, float dt,
+                 const float* rho, float* out, void* stream) {
+  if (!R || !S || !rho || !out || npt <= 0 || knt <= 0) return kfail(FW_ERR_ARG, "fwk_vscatter: bad argument");
+  const unsigned gy = (unsigned)((knt + NB - 1) / NB);
+  hipLaunchKernelGGL(k_vscatter, dim3(nblocks(npt, 256), gy), dim3(256), 0, (hipStream_t)stream, R, rcs, rts, S, scs,
+                     sts, npt, knt, dt, rho, out);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
+
+int fwk_eoshift(float* E, long cs, long ts, long npt, int knt, int ncomp, int shift, void* stream) {
+  if (!E || npt <= 0 || knt <= 0 || ncomp <= 0) return kfail(FW_ERR_ARG, "fwk_eoshift: bad argument");
+  hipLaunchKernelGGL(k_eoshift, dim3(nblocks(npt, 256)), dim3(256), 0, (hipStream_t)stream, E, cs, ts, npt, knt, ncomp,
+                     shift);
+  KCHK(hipGetLastError());
+  return FW_OK;
+}
+
+int fwk_reparam(int mode, long npt, const float* lam, const float* mu, float* Kap, float* Kaq, float* Kbp, float* Kbq,
+                void* stream) {
+  if (mode < 0 || mode > 2 || npt <= 0 || !lam || !mu || !Kap || !Kaq || !Kbp || !Kbq)
+    return kfail(FW_ERR_ARG, "fwk_reparam: bad argument");
+  hipLaunchKernelGGL(k_reparam, dim3(nblocks(npt, 256)), dim3(256), 0, (hipStream_t)stream, mode, npt, lam, mu, Kap,
+                     Kaq, Kbp, Kbq);
   KCHK(hipGetLastError());
   return FW_OK;
 }

@@ -8,7 +8,11 @@ transpose-free form of the reference's (i, j, k, n) arrays.  Work runs on
 the default stream; the synchronous copies of DeviceArray order it.
 
   stress2strain(E, lam, mu)                 SI_ker_sta.F90:1185-1211
-  sgt2scatter(R, S, lam, mu, dt)            :1257-1337 (one receiver component)
+  sgt2scatter(R, S, lam, mu, dt, param)     :1257-1337 (one receiver component;
+                                            param "lamu": SI_ker_sta_lamu.F90)
+  vel2scatter(R, S, rho, dt)                SI_ker_sta_rho.F90:1478-1552
+  eoshift(E, shift)                         SI_ker_EQ.F90:574-587
+  reparam(mode, lam, mu, Kap, Kaq, Kbp, Kbq)  SI_ker_{lamu2kamu,lamu2vpvs,vpvs2kamu}.F90
   scatter_filter(w0, b, a) / scatter_filtfilt   :1433-1475
   scatter2kernel(V, U, Sa, Sb, nt1, nt2, dt, M0)  :1375-1431
   scatter_rotate(X, Y, Z, rot)              :1354-1373
@@ -54,8 +58,10 @@ def stress2strain(E, lam, mu):
     return E
 
 
-def sgt2scatter(R, S, lam, mu, dt):
-    """alpha and beta scattered series (knt, npt) of one receiver component."""
+def sgt2scatter(R, S, lam, mu, dt, param="vpvs"):
+    """Scattered series (knt, npt) of one receiver component: ("vpvs")
+    alpha and beta (SI_ker_sta.F90:1290-1331) or ("lamu") lambda and mu
+    (SI_ker_sta_lamu.F90:1290-1336)."""
     L = _lib.lib()
     rp, rcs, rts, knt, npt = _series(R)
     sp, scs, sts, knt2, npt2 = _series(S)
@@ -63,9 +69,37 @@ def sgt2scatter(R, S, lam, mu, dt):
         raise ValueError("R and S shapes differ")
     A = DeviceArray((knt, npt))
     B = DeviceArray((knt, npt))
-    kcheck(L.fwk_scatter(rp, rcs, rts, sp, scs, sts, npt, knt, float(dt), _ptr(lam), _ptr(mu), _ptr(A), _ptr(B),
-                         _stream()), "fwk_scatter")
+    kcheck(L.fwk_scatter_param({"vpvs": 0, "lamu": 1}[param], rp, rcs, rts, sp, scs, sts, npt, knt, float(dt),
+                               _ptr(lam), _ptr(mu), _ptr(A), _ptr(B), _stream()), "fwk_scatter_param")
     return A, B
+
+
+def vel2scatter(R, S, rho, dt):
+    """Density scattered series (knt, npt) from 3-component velocity series
+    (SI_ker_sta_rho.F90:1478-1552)."""
+    L = _lib.lib()
+    rp, rcs, rts, knt, npt = _series(R)
+    sp, scs, sts, knt2, npt2 = _series(S)
+    if (knt, npt) != (knt2, npt2) or R.shape[0] < 3 or S.shape[0] < 3:
+        raise ValueError("R and S must be (3, knt, npt) series of the same shape")
+    out = DeviceArray((knt, npt))
+    kcheck(L.fwk_vscatter(rp, rcs, rts, sp, scs, sts, npt, knt, float(dt), _ptr(rho), _ptr(out), _stream()),
+           "fwk_vscatter")
+    return out
+
+
+def eoshift(E, shift):
+    """Fortran eoshift along time of a (ncomp, knt, npt) series in place (SI_ker_EQ.F90:574-587)."""
+    p, cs, ts, knt, npt = _series(E)
+    kcheck(_lib.lib().fwk_eoshift(p, cs, ts, npt, knt, E.shape[0], int(shift), _stream()), "fwk_eoshift")
+    return E
+
+
+def reparam(mode, lam, mu, Kap, Kaq, Kbp, Kbq):
+    """In-place kernel re-parameterisation: "lamu2kamu", "lamu2vpvs" or "vpvs2kamu"."""
+    m = {"lamu2kamu": 0, "lamu2vpvs": 1, "vpvs2kamu": 2}[mode]
+    kcheck(_lib.lib().fwk_reparam(m, Kap.numel(), _ptr(lam), _ptr(mu), _ptr(Kap), _ptr(Kaq), _ptr(Kbp), _ptr(Kbq),
+                                  _stream()), "fwk_reparam")
 
 
 def _filter(mode, w0, b, a):

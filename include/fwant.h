@@ -196,6 +196,27 @@ int fwk_stress2strain(float *E, long cs, long ts, long npt, int knt, const float
  * FP64 accumulation. */
 int fwk_scatter(const float *R, long rcs, long rts, const float *S, long scs, long sts, long npt, int knt,
                 float dt, const float *lam, const float *mu, float *outA, float *outB, void *stream);
+/* The same with the output pair selected: param 0 alpha/beta (as fwk_scatter),
+ * param 1 lambda/mu: outA = lambda*x1, outB = 2*mu*x2
+ * (SI_ker_sta_lamu.F90:1333-1336). */
+int fwk_scatter_param(int param, const float *R, long rcs, long rts, const float *S, long scs, long sts, long npt,
+                      int knt, float dt, const float *lam, const float *mu, float *outA, float *outB, void *stream);
+/* vel2scatter3 for one receiver component, SI_ker_sta_rho.F90:1478-1552:
+ * R and S are 3-component velocity series; out[knt][npt] = rho * dt *
+ * sum_{m<n} (VxR+VyR+VzR)(m) (VxS+VyS+VzS)(n-m).  The density kernel is
+ * then fwk_scatter2kernel with Sa = Sb = out (scatter2kernel_rho). */
+int fwk_vscatter(const float *R, long rcs, long rts, const float *S, long scs, long sts, long npt, int knt, float dt,
+                 const float *rho, float *out, void *stream);
+/* Fortran eoshift along time (dim 4) of ncomp series in place, zero fill
+ * (the earthquake-source SGT shift of SI_ker_EQ.F90:574-587). */
+int fwk_eoshift(float *E, long cs, long ts, long npt, int knt, int ncomp, int shift, void *stream);
+/* Kernel re-parameterisation in place on the phase (p) and amplitude (q)
+ * kernels of the first (a) and second (b) parameter: mode 0 lambda/mu ->
+ * kappa/mu (SI_ker_lamu2kamu.F90:823-830), 1 lambda/mu -> Vp/Vs
+ * (SI_ker_lamu2vpvs.F90:823-830), 2 Vp/Vs -> kappa/mu
+ * (SI_ker_vpvs2kamu.F90:823-831). */
+int fwk_reparam(int mode, long npt, const float *lam, const float *mu, float *Kap, float *Kaq, float *Kbp,
+                float *Kbq, void *stream);
 /* scatter_filter (mode 1, "P1") / scatter_filtfilt (mode 2, "P2"),
  * SI_ker_sta.F90:1433-1475; b, a are HOST arrays of nod1 doubles. w0, w: [knt][npt]. */
 int fwk_filter(int mode, long npt, int knt, int nod1, const double *b, const double *a, const float *w0,

@@ -1151,3 +1151,78 @@ void ORC(sgt_decode)(int fmt, int c0, int c1, int c2, long nslab, const uint16_t
 uint16_t ORC(f32_to_bf16)(float x) { return c_bf16_rne(x); }
 uint16_t ORC(f32_to_fp16)(float x) { return c_fp16_rne(x); }
 float ORC(fp16_to_f32)(uint16_t c) { return c_fp16_val(c); }
+
+/* ========================================================================
+ * Kernel variants (C14) and re-parameterisation (C15).
+ * ====================================================================== */
+/* sgt2scatter3 with lambda/mu outputs, SI_ker_sta_lamu.F90:1290-1336:
+ * outA = x1*lambda, outB = 2*x2*mu.  Pinned by
+ * tests/test_oracle_kernel_variants.py (numpy FP64 convolution). */
+void ORC(scatter1_lamu)(long npt, int knt, real dt, const real *const *R, const real *const *S, const real *lam,
+                        const real *miu, real *outA, real *outB) {
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < npt; p++) {
+    for (int n = 1; n <= knt; n++) {
+      real x1 = 0, x2 = 0;
+      for (int m = 1; m <= n - 1; m++) {
+        long qr = (long)(m - 1) * npt + p, qs = (long)(n - m - 1) * npt + p;
+        x1 = x1 + C_(1.0) * dt * (R[0][qr] + R[1][qr] + R[2][qr]) * (S[0][qs] + S[1][qs] + S[2][qs]);
+        x2 = x2 + C_(1.0) * dt *
+                      (R[0][qr] * S[0][qs] + R[1][qr] * S[1][qs] + R[2][qr] * S[2][qs] +
+                       C_(2.0) * R[3][qr] * S[3][qs] + C_(2.0) * R[4][qr] * S[4][qs] + C_(2.0) * R[5][qr] * S[5][qs]);
+      }
+      long qo = (long)(n - 1) * npt + p;
+      outA[qo] = x1 * lam[p];
+      outB[qo] = C_(2.0) * x2 * miu[p];
+    }
+  }
+}
+/* vel2scatter3, SI_ker_sta_rho.F90:1478-1552 (one receiver component):
+ * R[3], S[3] velocity series; out = rho * dt * sum (sum_c R_c)(m) (sum_c S_c)(n-m). */
+void ORC(vscatter1)(long npt, int knt, real dt, const real *const *R, const real *const *S, const real *rho,
+                    real *out) {
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < npt; p++) {
+    for (int n = 1; n <= knt; n++) {
+      real x1 = 0;
+      for (int m = 1; m <= n - 1; m++) {
+        long qr = (long)(m - 1) * npt + p, qs = (long)(n - m - 1) * npt + p;
+        x1 = x1 + C_(1.0) * dt * (R[0][qr] + R[1][qr] + R[2][qr]) * (S[0][qs] + S[1][qs] + S[2][qs]);
+      }
+      out[(long)(n - 1) * npt + p] = x1 * C_(1.0) * rho[p];
+    }
+  }
+}
+/* eoshift along time of one [knt][npt] series, Fortran semantics (shift > 0:
+ * out(n) = in(n+shift), zero beyond the end), SI_ker_EQ.F90:574-587. */
+void ORC(eoshift)(long npt, int knt, int shift, const real *in, real *out) {
+  for (int n = 0; n < knt; n++)
+    for (long p = 0; p < npt; p++) {
+      int s = n + shift;
+      out[(long)n * npt + p] = (s >= 0 && s < knt) ? in[(long)s * npt + p] : C_(0.0);
+    }
+}
+/* re-parameterisation of (a, b) kernel pairs, FP32 order of operations of
+ * SI_ker_lamu2kamu.F90:823-830 (mode 0), SI_ker_lamu2vpvs.F90:823-830
+ * (mode 1), SI_ker_vpvs2kamu.F90:823-831 (mode 2).  Pinned by the
+ * chain-rule invariance test in tests/test_oracle_kernel_variants.py. */
+void ORC(reparam)(int mode, long npt, const real *lam, const real *mu, real *a, real *b) {
+  for (long p = 0; p < npt; p++) {
+    real l = lam[p], u = mu[p], x = a[p], y = b[p];
+    if (mode == 0) {
+      x = x / l; y = y / u / C_(2.0);
+      y = (C_(2.0) * y - C_(2.0) * x / C_(3.0)) * u;
+      x = x * (l + C_(2.0) * u / C_(3.0));
+    } else if (mode == 1) {
+      x = x / l; y = y / u / C_(2.0);
+      y = C_(2.0) * (y - x) * C_(2.0) * u;
+      x = x * C_(2.0) * (l + C_(2.0) * u);
+    } else {
+      x = x / (C_(2.0) * (l + C_(2.0) * u));
+      y = y / u / C_(4.0) + x;
+      y = (C_(2.0) * y - C_(2.0) * x / C_(3.0)) * u;
+      x = x * (l + C_(2.0) * u / C_(3.0));
+    }
+    a[p] = x; b[p] = y;
+  }
+}

@@ -57,6 +57,10 @@ def _load(prec):
         "series_filter": (None, [C.c_int, C.c_long, C.c_int, C.c_int, dp, dp, rp, rp]),
         "scatter2kernel": (None, [C.c_long, C.c_int, C.c_int, C.c_int, R, R, rp, rp, rp, rp, rp, rp, rp, rp]),
         "scatter_rotate": (None, [C.c_long, C.c_int, rp, rp, rp, rp]),
+        "scatter1_lamu": (None, [C.c_long, C.c_int, R, C.POINTER(rp), C.POINTER(rp), rp, rp, rp, rp]),
+        "vscatter1": (None, [C.c_long, C.c_int, R, C.POINTER(rp), C.POINTER(rp), rp, rp]),
+        "eoshift": (None, [C.c_long, C.c_int, C.c_int, rp, rp]),
+        "reparam": (None, [C.c_int, C.c_long, rp, rp, rp, rp]),
         "sgt_encode": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp, vp]),
         "sgt_decode": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp, vp]),
         "f32_to_bf16": (C.c_uint16, [C.c_float]),
