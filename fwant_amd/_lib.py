@@ -22,7 +22,8 @@ SYMBOLS = (
     "fw_init", "fw_get_pml_profiles", "fw_step", "fw_run", "fw_reference_steps", "fw_sync",
     "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_add_snapshot_packed",
     "fw_get_snapshot_packed", "fw_snapshot_packed_device", "fwk_sgt_decode", "fw_set_batch",
-    "fw_select_member", "fw_set_batch_width", "fw_use_graphs", "fw_autotune", "fw_get_wavefield",
+    "fw_select_member", "fw_set_batch_width", "fw_use_graphs", "fw_autotune", "fw_set_cycle_origin",
+    "fw_checkpoint_size", "fw_checkpoint_save", "fw_checkpoint_load", "fw_get_wavefield",
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_scatter_param", "fwk_vscatter", "fwk_eoshift", "fwk_reparam",
     "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
@@ -87,6 +88,10 @@ def _sig(lib):
         "fw_set_batch_width": (C.c_int, [H, C.c_int]),
         "fw_use_graphs": (C.c_int, [H, C.c_int]),
         "fw_autotune": (C.c_int, [H, C.c_int, ip, ip, fp]),
+        "fw_set_cycle_origin": (C.c_int, [H, C.c_int]),
+        "fw_checkpoint_size": (C.c_int, [H, C.POINTER(C.c_size_t)]),
+        "fw_checkpoint_save": (C.c_int, [H, vp]),
+        "fw_checkpoint_load": (C.c_int, [H, vp]),
         "fw_get_wavefield": (C.c_int, [H, C.c_int, fp]),
         "fw_set_wavefield": (C.c_int, [H, C.c_int, fp]),
         "fw_enable_timing": (C.c_int, [H, C.c_int]),

@@ -453,6 +453,7 @@ struct fw_solver_s {
   long bstride = 0;                   // floats between members' register sets
   int width = 0;                      // members per stage launch (0: all running)
   int sel = 0;                        // member addressed by the get/set calls
+  int cycle_origin = 0;               // ntime at which the 8-step F/B cycle starts
   std::vector<int> rec_cnt;           // receiver records held by each slot
   int* d_clk = nullptr;               // device step clock: ntime, record counters
   int* d_srow = nullptr;              // member slot of each source row
@@ -1177,7 +1178,7 @@ static int build_items(fw_handle h) {
 extern "C" {
 
 static int launch_stage(fw_handle h, int ntime, int st, int nact) {
-  const int* bp = BLOCK_PAT[((ntime % 8) + 8) % 8];
+  const int* bp = BLOCK_PAT[(((ntime - h->cycle_origin) % 8) + 8) % 8];
   const int flip = st & 1;
   const int xf = flip ? !bp[0] : bp[0], yf = flip ? !bp[1] : bp[1], zf = flip ? !bp[2] : bp[2];
   const int pat = (xf << 2) | (yf << 1) | zf;
@@ -1344,7 +1345,7 @@ static int enqueue_step(fw_handle h, int ntime, int nact, bool all) {
 // The step as a graph: launches depend on ntime only through its block
 // position (the F/B pattern); times and record slots come from the clock.
 static int step_graph(fw_handle h, int ntime, int nact, hipGraphExec_t* out) {
-  const long key = (long)(ntime % 8) * 4096 + nact;
+  const long key = (long)((((ntime - h->cycle_origin) % 8) + 8) % 8) * 4096 + nact;
   auto it = h->gexec.find(key);
   if (it != h->gexec.end()) {
     *out = it->second;
@@ -1538,6 +1539,66 @@ int fw_set_wavefield(fw_handle h, int f, const float* in) {
   int rc = upload_volume(h, h->set[0] + (size_t)sel_slot(h) * h->bstride + (size_t)f * h->fs, in);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
+  return FW_OK;
+}
+
+int fw_set_cycle_origin(fw_handle h, int origin) {
+  if (!h) return FW_ERR_ARG;
+  h->cycle_origin = origin;
+  return FW_OK;
+}
+
+// Exact-restart state of the selected member: register set A (the
+// wavefield between steps) and set A of every CFS-PML face's auxiliaries.
+static size_t checkpoint_floats(fw_handle h) {
+  size_t n = (size_t)9 * h->nx * h->ny * h->nz;
+  for (const auto& f : h->face)
+    if (f.on) n += 6 * (size_t)f.cnt;
+  return n;
+}
+
+int fw_checkpoint_size(fw_handle h, size_t* bytes) {
+  if (!h || !bytes) return FW_ERR_ARG;
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_checkpoint_size: call fw_init first");
+  *bytes = sizeof(float) * checkpoint_floats(h);
+  return FW_OK;
+}
+
+int fw_checkpoint_save(fw_handle h, void* buf) {
+  if (!h || !buf) return fail(h, FW_ERR_ARG, "fw_checkpoint_save: null argument");
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_checkpoint_save: call fw_init first");
+  float* o = (float*)buf;
+  const size_t vol = (size_t)h->nx * h->ny * h->nz;
+  for (int f = 0; f < 9; ++f) {
+    int rc = fw_get_wavefield(h, f, o + f * vol);
+    if (rc) return rc;
+  }
+  o += 9 * vol;
+  const int sl = sel_slot(h);
+  for (const auto& f : h->face) {
+    if (!f.on) continue;
+    HIPCHK(hipMemcpy(o, f.d + (size_t)sl * 24 * f.cnt, sizeof(float) * 6 * f.cnt, hipMemcpyDeviceToHost));
+    o += 6 * f.cnt;
+  }
+  return FW_OK;
+}
+
+int fw_checkpoint_load(fw_handle h, const void* buf) {
+  if (!h || !buf) return fail(h, FW_ERR_ARG, "fw_checkpoint_load: null argument");
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_checkpoint_load: call fw_init first");
+  const float* in = (const float*)buf;
+  const size_t vol = (size_t)h->nx * h->ny * h->nz;
+  for (int f = 0; f < 9; ++f) {
+    int rc = fw_set_wavefield(h, f, in + f * vol);
+    if (rc) return rc;
+  }
+  in += 9 * vol;
+  const int sl = sel_slot(h);
+  for (const auto& f : h->face) {
+    if (!f.on) continue;
+    HIPCHK(hipMemcpy(f.d + (size_t)sl * 24 * f.cnt, in, sizeof(float) * 6 * f.cnt, hipMemcpyHostToDevice));
+    in += 6 * f.cnt;
+  }
   return FW_OK;
 }
 

@@ -361,3 +361,33 @@ def write_snapshots(root, snap_no, snap, tcnt, records, stept):
                     v[:cnt] = records[first:first + cnt, q].reshape(cnt, subc[2], subc[1], subc[0])
             paths.append(path)
     return paths
+
+
+def write_rest(root, ntime, fields, state=None, cycle_origin=None):
+    """rest_tNNNNN_mpi000000.nc (io_rest_export, mod_io.F90:376-399): the 9
+    ghosted wavefields (dims I, J, K).  With `state` (Solver.checkpoint())
+    the file also carries the complete state for an exact restart, which
+    the reference's reader ignores."""
+    path = os.path.join(root, f"rest_t{ntime:05d}_{SUFFIX}.nc")
+    nx, ny, nz = np.asarray(fields["Vx"]).shape
+    with netcdf_file(path, "w", version=2) as nc:
+        nc.title = "Restart file generated by seis3d_wave"
+        for d, v in zip("IJK", (nx, ny, nz)):
+            nc.createDimension(d, v)
+        for name in SEISMO_VARS:
+            nc.createVariable(name, "f4", ("K", "J", "I"))[:] = np.asarray(fields[name], np.float32).T
+        if state is not None:
+            nc.createDimension("state", len(state))
+            nc.createVariable("fwant_state", "f4", ("state",))[:] = np.asarray(state, np.float32)
+            nc.cycle_origin = np.int32(0 if cycle_origin is None else cycle_origin)
+        nc.ntime = np.int32(ntime)
+    return path
+
+
+def read_rest(path):
+    """(fields, state or None, cycle origin or None) of a rest_ file."""
+    with netcdf_file(path, "r", mmap=False) as nc:
+        fields = {n: _fortran3(np.array(nc.variables[n][:])) for n in SEISMO_VARS}
+        state = np.array(nc.variables["fwant_state"][:]) if "fwant_state" in nc.variables else None
+        origin = int(nc.cycle_origin) if hasattr(nc, "cycle_origin") else None
+    return fields, state, origin

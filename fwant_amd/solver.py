@@ -129,6 +129,27 @@ class Solver:
         """Address member `member` of a batch in the output / wavefield calls."""
         check(self.L.fw_select_member(self.h, int(member)), self.h, "fw_select_member")
 
+    def set_cycle_origin(self, origin):
+        """Step ntime uses F/B block (ntime - origin) mod 8 (a reference-style
+        restart at ntime0 passes origin = ntime0)."""
+        check(self.L.fw_set_cycle_origin(self.h, int(origin)), self.h, "fw_set_cycle_origin")
+
+    def checkpoint(self):
+        """Complete state between steps (wavefield + PML auxiliaries) as a float32 array."""
+        n = C.c_size_t()
+        check(self.L.fw_checkpoint_size(self.h, C.byref(n)), self.h, "fw_checkpoint_size")
+        buf = np.zeros(n.value // 4, np.float32)
+        check(self.L.fw_checkpoint_save(self.h, buf.ctypes.data_as(C.c_void_p)), self.h, "fw_checkpoint_save")
+        return buf
+
+    def restore(self, buf):
+        buf = np.ascontiguousarray(buf, np.float32)
+        n = C.c_size_t()
+        check(self.L.fw_checkpoint_size(self.h, C.byref(n)), self.h, "fw_checkpoint_size")
+        if buf.nbytes != n.value:
+            raise ValueError(f"checkpoint holds {buf.nbytes} bytes, this solver needs {n.value}")
+        check(self.L.fw_checkpoint_load(self.h, buf.ctypes.data_as(C.c_void_p)), self.h, "fw_checkpoint_load")
+
     def use_graphs(self, on=True):
         check(self.L.fw_use_graphs(self.h, 1 if on else 0), self.h, "fw_use_graphs")
 

@@ -177,6 +177,20 @@ int fw_use_graphs(fw_handle h, int on);
  * zero initial state with empty records.  Call after fw_init. */
 int fw_autotune(fw_handle h, int trial_steps, int *tile, int *width, float *ms_per_step);
 
+/* Restart.  The reference restarts from the 9 wavefields alone: its PML
+ * memory variables start again at zero and the 8-step F/B cycle starts
+ * again at its first block (seis3d_wave_8op_p3.F90:72-95, mod_io.F90:
+ * 376-427).  fw_set_wavefield + fw_set_cycle_origin(h, ntime0) reproduce
+ * that.  fw_checkpoint_* (no counterpart in the reference) save and load
+ * the complete state between steps (wavefield and PML auxiliaries of the
+ * selected member): a run continued from a checkpoint with the original
+ * cycle origin is identical to an uninterrupted run.  Pattern of step
+ * ntime: block (ntime - origin) mod 8; default origin 0. */
+int fw_set_cycle_origin(fw_handle h, int origin);
+int fw_checkpoint_size(fw_handle h, size_t *bytes);
+int fw_checkpoint_save(fw_handle h, void *buf);
+int fw_checkpoint_load(fw_handle h, const void *buf);
+
 /* Per-kernel timing of the last fw_run (milliseconds of the main stage
  * kernels summed, measured with HIP events on the handle's stream). */
 int fw_enable_timing(fw_handle h, int on);

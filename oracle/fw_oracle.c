@@ -124,6 +124,7 @@ typedef struct {
   int npt, pt_tinv;
   int *pt_indx; /* 3*npt C indices */
   int nthreads;
+  int cycle_origin; /* ntime at which the 8-step F/B cycle starts (a restarted run starts it anew) */
 } orc_solver;
 
 #define IDX(s, i, j, k) ((long)(i) + (s)->sy * (long)(j) + (s)->sz * (long)(k))
@@ -887,7 +888,7 @@ void ORC(step)(orc_solver *s, int ntime) {
   for (int st = 0; st < 4; st++) {
     real tinc = st == 0 ? C_(0.0) : rka(st - 1);
     int pat[3];
-    ORC(stage_pattern)(ntime, st, pat);
+    ORC(stage_pattern)(ntime - s->cycle_origin, st, pat);
     src_surface(s, ntime, tinc);
     free_timg(s);
     op_sweep(s, pat[0], pat[1], pat[2]);
@@ -922,6 +923,8 @@ int ORC(run)(orc_solver *s, int ntime0, int nsteps, real *rec) {
   return nrec;
 }
 
+/* a restarted run re-enters the cycle at its first block, seis3d_wave_8op_p3.F90:72-95 */
+void ORC(set_cycle_origin)(orc_solver *s, int origin) { s->cycle_origin = origin; }
 void ORC(get_field)(orc_solver *s, int f, real *out) { memcpy(out, s->W[f], sizeof(real) * s->ncell); }
 void ORC(set_field)(orc_solver *s, int f, const real *in) { memcpy(s->W[f], in, sizeof(real) * s->ncell); }
 long ORC(ncell)(orc_solver *s) { return s->ncell; }

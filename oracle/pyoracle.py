@@ -43,6 +43,7 @@ def _load(prec):
         "step": (None, [vp, C.c_int]),
         "run": (C.c_int, [vp, C.c_int, C.c_int, rp]),
         "get_field": (None, [vp, C.c_int, rp]),
+        "set_cycle_origin": (None, [vp, C.c_int]),
         "set_field": (None, [vp, C.c_int, rp]),
         "ncell": (C.c_long, [vp]),
         "rhs": (None, [vp, C.c_int, C.c_int, C.c_int, rp]),
@@ -194,6 +195,9 @@ class OracleSolver:
         if not self.records:
             return np.zeros((0, self.npt, 9), self.lib.dtype)
         return np.concatenate(self.records, 0)
+
+    def set_cycle_origin(self, origin):
+        self.lib.set_cycle_origin(self.h, int(origin))
 
     def wavefield(self, f):
         from fwant_amd._lib import FIELDS
