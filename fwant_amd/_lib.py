@@ -27,7 +27,7 @@ SYMBOLS = (
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_scatter_param", "fwk_vscatter", "fwk_eoshift", "fwk_reparam",
     "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
-    "fwk_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
+    "fwk_last_error", "fwk_assemble", "fwl_create", "fwl_solve", "fwl_destroy", "fwl_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
     "fw_memcpy_d2d", "fw_device_sync",
 )
 
@@ -110,6 +110,12 @@ def _sig(lib):
                                          vp, vp, vp, vp, vp, vp, vp]),
         "fwk_rotate": (C.c_int, [C.c_long, vp, vp, vp, fp, vp]),
         "fwk_last_error": (C.c_char_p, []),
+        "fwk_assemble": (C.c_int, [C.c_long, vp, C.c_long, vp, vp, vp, vp, vp]),
+        "fwl_create": (C.c_int, [C.c_long, C.c_long, C.c_long, vp, vp, vp, vp, C.POINTER(vp)]),
+        "fwl_solve": (C.c_int, [vp, C.c_double, C.c_double, C.c_long, vp, vp, C.POINTER(C.c_long),
+                                C.POINTER(C.c_double)]),
+        "fwl_destroy": (C.c_int, [vp]),
+        "fwl_last_error": (C.c_char_p, [vp]),
         "fw_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(vp)]),
         "fw_device_free": (C.c_int, [vp]),
         "fw_memcpy_h2d": (C.c_int, [vp, vp, C.c_size_t]),

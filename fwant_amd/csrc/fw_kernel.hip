@@ -21,6 +21,10 @@ int kfail(int code, const std::string& m) {
   g_kerr = m;
   return code;
 }
+}  // namespace
+// shared with the other fwk_* sources (fw_inv.hip)
+int fwk_fail(int code, const std::string& m) { return kfail(code, m); }
+namespace {
 #define KCHK(expr)                                                                       \
   do {                                                                                   \
     hipError_t e_ = (expr);                                                              \

@@ -247,8 +247,30 @@ int fwk_rotate(long n, float *X, float *Y, float *Z, const float *rot, void *str
  * snapshot, codes and scales on the device, out[nslab][npts]. */
 int fwk_sgt_decode(int format, const uint16_t *codes, const float *scales, int c0, int c1, int c2, long nslab,
                    float *out, void *stream);
+/* Assembly of a kernel onto inversion blocks, SI_ker2assm.F90:300-320:
+ * B[n] = sum_{q in [blk_ptr[n], blk_ptr[n+1])} K[pix[q]] * vol[q] (FP64
+ * accumulation).  K, blk_ptr, pix (0-based point indices), vol and B are
+ * DEVICE arrays (the CSR form of indx_of_pix / num_of_pix / volume). */
+int fwk_assemble(long npt, const float *K, long nblk, const long *blk_ptr, const long *pix, const float *vol,
+                 double *B, void *stream);
 /* Last error of the fwk_* family and of the device-memory helpers below. */
 const char *fwk_last_error(void);
+
+/* ---------------------------------------------------------------------------
+ * Damped least squares by LSQR on the GPU (inv.LSQR/mod_LSQR.f90:142-292,
+ * Paige & Saunders, FP64): A in CSR (the rows lldrow adds: data rows,
+ * station/event terms, damping, smoothing; 0-based columns), rhs b.
+ * fwl_solve runs slsqr's iteration with its stopping tests (test1 =
+ * rnorm/(|A|_F |x|) <= atol, test2 = alpha|c|/|A|_F <= btol, or maxit) and
+ * returns x, the standard errors of lgtsol (err may be NULL), the
+ * iteration count and the final residual norm.  Host arrays.
+ * ------------------------------------------------------------------------- */
+typedef struct fwl_s *fwl_handle;
+int fwl_create(long nrow, long ncol, long nnz, const long *row_ptr, const long *col, const double *val,
+               const double *rhs, fwl_handle *out);
+int fwl_solve(fwl_handle h, double atol, double btol, long maxit, double *x, double *err, long *itct, double *res);
+int fwl_destroy(fwl_handle h);
+const char *fwl_last_error(fwl_handle h);
 
 /* Device-memory helpers for hosts without their own HIP allocator (the
  * reference keeps these arrays in Fortran allocatables; a Fortran or Python

@@ -1229,3 +1229,85 @@ void ORC(reparam)(int mode, long npt, const real *lam, const real *mu, real *a, 
     a[p] = x; b[p] = y;
   }
 }
+
+/* ========================================================================
+ * Inversion (C15/C16): block assembly and LSQR, always in double.
+ * ====================================================================== */
+/* SI_ker2assm.F90:300-320: B(n) = sum over block n's pixels of K * volume */
+void ORC(assemble)(long nblk, const long *ptr, const long *pix, const float *vol, const float *K, double *B) {
+  for (long n = 0; n < nblk; n++) {
+    double s = 0.0;
+    for (long q = ptr[n]; q < ptr[n + 1]; q++) s += (double)K[pix[q]] * (double)vol[q];
+    B[n] = s;
+  }
+}
+/* slsqr + lgtsol, inv.LSQR/mod_LSQR.f90:142-292, on CSR rows (lldrow's
+ * ra/ja/na storage with 0-based columns).  Pinned by
+ * tests/test_oracle_inversion.py against numpy.linalg.lstsq. */
+static double l_dot(const double *a, const double *b, long n) {
+  double s = 0.0;
+  for (long i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+static double l_nrliz(double *a, long n) {
+  double v0 = sqrt(l_dot(a, a, n));
+  if (fabs(v0) <= 1.0e-20) return v0;
+  for (long i = 0; i < n; i++) a[i] = a[i] / v0;
+  return v0;
+}
+long ORC(lsqr)(long m, long n, const long *ptr, const long *col, const double *val, const double *b, double atol,
+               double btol, long maxit, double *x, double *err, double *res) {
+  double *u = calloc(m, sizeof(double)), *v = calloc(n, sizeof(double)), *w = calloc(n, sizeof(double)),
+         *q = calloc(m > n ? m : n, sizeof(double)), *sig = calloc(n, sizeof(double));
+  double anorm = 0.0;
+  for (long k = 0; k < ptr[m]; k++) anorm += val[k] * val[k];
+  anorm = sqrt(anorm);
+  for (long i = 0; i < m; i++) u[i] = b[i];
+  double beta = l_nrliz(u, m);
+  for (long i = 0; i < n; i++) v[i] = 0.0;
+  for (long r = 0; r < m; r++)
+    for (long k = ptr[r]; k < ptr[r + 1]; k++) v[col[k]] += val[k] * u[r];
+  double alpha = l_nrliz(v, n);
+  for (long i = 0; i < n; i++) { w[i] = v[i]; x[i] = 0.0; sig[i] = 0.0; }
+  double phibar = beta, rhobar = alpha, res2 = 0.0;
+  long itct = 0;
+  for (;;) {
+    for (long r = 0; r < m; r++) {
+      double y0 = 0.0;
+      for (long k = ptr[r]; k < ptr[r + 1]; k++) y0 += val[k] * v[col[k]];
+      q[r] = y0;
+    }
+    for (long i = 0; i < m; i++) u[i] = q[i] - u[i] * alpha;
+    beta = l_nrliz(u, m);
+    for (long i = 0; i < n; i++) q[i] = 0.0;
+    for (long r = 0; r < m; r++)
+      for (long k = ptr[r]; k < ptr[r + 1]; k++) q[col[k]] += val[k] * u[r];
+    for (long i = 0; i < n; i++) v[i] = q[i] - v[i] * beta;
+    alpha = l_nrliz(v, n);
+    double rho = sqrt(rhobar * rhobar + beta * beta);
+    if (rho == 0.0) break;
+    double c = rhobar / rho, s = beta / rho, theta = s * alpha;
+    rhobar = -c * alpha;
+    double phi = c * phibar;
+    phibar = s * phibar;
+    for (long i = 0; i < n; i++) {
+      double t = w[i] / rho;
+      x[i] = x[i] + phi * t;
+      w[i] = v[i] - theta * t;
+      sig[i] = sig[i] + t * t;
+    }
+    double test1 = phibar / anorm / sqrt(l_dot(x, x, n));
+    double test2 = alpha * fabs(c) / anorm;
+    itct++;
+    if (test1 <= atol || test2 <= btol || itct == maxit) {
+      res2 = phibar;
+      break;
+    }
+  }
+  double temp = res2 / (double)(m - n > 1 ? m - n : 1);
+  if (err)
+    for (long i = 0; i < n; i++) err[i] = sqrt(temp * sig[i]);
+  if (res) *res = res2;
+  free(u); free(v); free(w); free(q); free(sig);
+  return itct;
+}

@@ -62,6 +62,8 @@ def _load(prec):
         "vscatter1": (None, [C.c_long, C.c_int, R, C.POINTER(rp), C.POINTER(rp), rp, rp]),
         "eoshift": (None, [C.c_long, C.c_int, C.c_int, rp, rp]),
         "reparam": (None, [C.c_int, C.c_long, rp, rp, rp, rp]),
+        "assemble": (None, [C.c_long, vp, vp, vp, vp, vp]),
+        "lsqr": (C.c_long, [C.c_long, C.c_long, vp, vp, vp, vp, C.c_double, C.c_double, C.c_long, vp, vp, vp]),
         "sgt_encode": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp, vp]),
         "sgt_decode": (None, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_long, vp, vp, vp]),
         "f32_to_bf16": (C.c_uint16, [C.c_float]),
