@@ -27,7 +27,8 @@ SYMBOLS = (
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_scatter_param", "fwk_vscatter", "fwk_eoshift", "fwk_reparam",
     "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
-    "fwk_last_error", "fwk_assemble", "fwl_create", "fwl_solve", "fwl_destroy", "fwl_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
+    "fwk_last_error", "fwae_conv_fwd", "fwae_conv_bwd_data", "fwae_conv_bwd_weight", "fwae_act_fwd", "fwae_act_bwd",
+    "fwae_mse", "fwae_adam", "fwae_blocks", "fwk_assemble", "fwl_create", "fwl_solve", "fwl_destroy", "fwl_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
     "fw_memcpy_d2d", "fw_device_sync",
 )
 
@@ -110,6 +111,14 @@ def _sig(lib):
                                          vp, vp, vp, vp, vp, vp, vp]),
         "fwk_rotate": (C.c_int, [C.c_long, vp, vp, vp, fp, vp]),
         "fwk_last_error": (C.c_char_p, []),
+        "fwae_conv_fwd": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, vp, vp]),
+        "fwae_conv_bwd_data": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, vp, vp]),
+        "fwae_conv_bwd_weight": (C.c_int, [C.c_int] * 8 + [vp, vp, C.c_int, vp, vp, vp]),
+        "fwae_act_fwd": (C.c_int, [C.c_long, C.c_float, vp, vp]),
+        "fwae_act_bwd": (C.c_int, [C.c_long, C.c_float, vp, vp, vp]),
+        "fwae_mse": (C.c_int, [C.c_int, C.c_long, vp, vp, vp, vp, vp]),
+        "fwae_adam": (C.c_int, [C.c_long, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int, vp]),
+        "fwae_blocks": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp]),
         "fwk_assemble": (C.c_int, [C.c_long, vp, C.c_long, vp, vp, vp, vp, vp]),
         "fwl_create": (C.c_int, [C.c_long, C.c_long, C.c_long, vp, vp, vp, vp, C.POINTER(vp)]),
         "fwl_solve": (C.c_int, [vp, C.c_double, C.c_double, C.c_long, vp, vp, C.POINTER(C.c_long),

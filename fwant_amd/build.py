@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfwant.so")
-SOURCES = ["fw_solver.hip", "fw_kernel.hip", "fw_inv.hip"]
+SOURCES = ["fw_solver.hip", "fw_kernel.hip", "fw_inv.hip", "fw_ae.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [

@@ -257,6 +257,32 @@ int fwk_assemble(long npt, const float *K, long nblk, const long *blk_ptr, const
 const char *fwk_last_error(void);
 
 /* ---------------------------------------------------------------------------
+ * SGT block autoencoder (no counterpart in the reference): primitives of a
+ * 3-D convolutional network on 16^3 blocks, trained and run on the GPU.
+ * Device arrays, NCDHW layout, cubic kernels (K, stride S, padding P,
+ * Do = (Di + 2P - K)/S + 1).  fwae_conv_bwd_data adds b2[ci] when b2 is not
+ * NULL (the bias of a transposed convolution, whose forward pass it is).
+ * fwae_conv_bwd_weight writes dw [Co][Ci][K^3] then db [Co] into dwb,
+ * using part (nchunk * (Co*Ci*K^3 + Co) floats) for a two-pass reduction.
+ * fwae_act_fwd applies leaky ReLU (slope; 1 = identity) and rounds the
+ * stored activation to BF16.  fwae_blocks cuts snapshot records
+ * [nrec][nc][c2][c1][c0] into [nrec][nblk][nc][16^3] blocks scaled to
+ * [-1, 1] per block and component (scale [nrec][nblk][nc]).
+ * ------------------------------------------------------------------------- */
+int fwae_conv_fwd(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float *x, const float *w,
+                  const float *b, float *y, void *stream);
+int fwae_conv_bwd_data(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float *dy, const float *w,
+                       const float *b2, float *dx, void *stream);
+int fwae_conv_bwd_weight(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float *x, const float *dy,
+                         int nchunk, float *part, float *dwb, void *stream);
+int fwae_act_fwd(long n, float slope, float *y, void *stream);
+int fwae_act_bwd(long n, float slope, const float *y, float *dy, void *stream);
+int fwae_mse(int N, long per, const float *y, const float *t, float *grad, double *loss, void *stream);
+int fwae_adam(long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps, int step,
+              void *stream);
+int fwae_blocks(const float *rec, int nrec, int nc, int c0, int c1, int c2, float *out, float *scale, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Damped least squares by LSQR on the GPU (inv.LSQR/mod_LSQR.f90:142-292,
  * Paige & Saunders, FP64): A in CSR (the rows lldrow adds: data rows,
  * station/event terms, damping, smoothing; 0-based columns), rhs b.

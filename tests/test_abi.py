@@ -14,7 +14,7 @@ LIB = os.path.join(ROOT, "fwant_amd", "libfwant.so")
 def header_functions():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(fw[kl]?_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(fw(?:k|l|ae)?_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_parses():
