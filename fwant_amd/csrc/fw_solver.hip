@@ -1059,6 +1059,7 @@ int fw_init(fw_handle h) {
     if (n % 2 == 0) f.hi[ax] = f.lo[ax] + N; else f.lo[ax] = f.hi[ax] - N;
     f.cnt = 1;
     for (int d = 0; d < 3; ++d) { f.ex[d] = f.hi[d] - f.lo[d] + 1; f.cnt *= f.ex[d]; }
+    if (f.cnt >= (1L << 31)) return fail(h, FW_ERR_ARG, "CFS-PML slab with 2^31 or more cells");
     HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt * h->nbatch));
   }
   int rc = build_items(h);

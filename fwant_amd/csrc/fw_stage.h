@@ -94,32 +94,94 @@ struct StageArgs {
 __device__ __forceinline__ int member_slot(const StageArgs& g) { return g.slot0 + (int)blockIdx.y; }
 
 __device__ __forceinline__ long face_index(const FaceArgs& f, int i, int j, int k) {
-  return (long)(i - f.lo0) + (long)f.ex0 * ((j - f.lo1) + (long)f.ex1 * (k - f.lo2));
+  // a face holds fewer than 2^31 cells (checked at set-up)
+  return (long)((i - f.lo0) + f.ex0 * ((j - f.lo1) + f.ex1 * (k - f.lo2)));
 }
 __device__ __forceinline__ bool in_face(const FaceArgs& f, int i, int j, int k) {
   return f.on && i >= f.lo0 && i <= f.hi0 && j >= f.lo1 && j <= f.hi1 && k >= f.lo2 && k <= f.hi2;
 }
-// auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105)
 __device__ __forceinline__ float* aux_base(const FaceArgs& f, const StageArgs& g) {
   return f.d + (long)member_slot(g) * 24 * f.cnt;
 }
-template <int KIND>
-__device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, long li, int v, float psi, float dpsi) {
+
+// Auxiliary values of three variables of one face cell, loaded ahead of
+// use: psi of the stage's input set and, when FULL, the set-A and set-T
+// values of the update (by RK kind).
+struct AuxVals {
+  float p[3], a[3], t[3];
+};
+__device__ __forceinline__ float sel3(int m, const float v[3]) { return m == 0 ? v[0] : (m == 1 ? v[1] : v[2]); }
+
+// auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105);
+// pre: the set-A / set-T values were loaded ahead (pa, pt)
+template <int KIND, bool FULL>
+__device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, long li, int v, float psi, float dpsi,
+                                       bool pre, float pa, float pt) {
   const long S = 6 * f.cnt, q = v * f.cnt + li;
   float* const d = aux_base(f, g);
   if constexpr (KIND == 0) {
     d[g.setout * S + q] = psi + g.a * dpsi;
     d[3 * S + q] = psi + g.b * dpsi;
   } else if constexpr (KIND == 1) {
-    d[g.setout * S + q] = d[q] + g.a * dpsi;
-    d[3 * S + q] = d[3 * S + q] + g.b * dpsi;
+    const float A = FULL && pre ? pa : d[q], T = FULL && pre ? pt : d[3 * S + q];
+    d[g.setout * S + q] = A + g.a * dpsi;
+    d[3 * S + q] = T + g.b * dpsi;
   } else {
-    d[g.setout * S + q] = d[3 * S + q] + g.b * dpsi;
+    const float T = FULL && pre ? pt : d[3 * S + q];
+    d[g.setout * S + q] = T + g.b * dpsi;
   }
 }
 __device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, long li, int v) {
   return aux_base(f, g)[g.setin * 6 * f.cnt + v * f.cnt + li];
 }
+
+// Prefetch of the auxiliary values of the first x or y face slab holding a
+// cell.  x and y slabs span every plane of the volume, so the face and its
+// index step per plane are fixed for the column; the loads of plane k are
+// issued with the RK operands, before the plane's barrier.  Cells in more
+// than one slab (edges, corners) load the other slabs' values where they are
+// applied.
+struct AuxPre {
+  const float* base;  // set A, first of the three variables, this cell, plane k0
+  int face;           // 0..3, or -1: no x or y slab
+  int cnt, kst;       // face cell count, index step per plane
+  int mask;           // bit n: the cell is in x or y slab n (0..3)
+};
+__device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int i, int j, int k0, int v0, bool comp) {
+  AuxPre r{nullptr, -1, 0, 0, 0};
+  for (int n = 0; n < 4; ++n) {
+    const FaceArgs& f = g.face[n];
+    if (comp && in_face(f, i, j, k0)) {
+      r.mask |= 1 << n;
+      if (r.face < 0) {
+        r.face = n;
+        r.cnt = (int)f.cnt;
+        r.kst = f.ex0 * f.ex1;
+        r.base = aux_base(f, g) + (long)v0 * f.cnt + face_index(f, i, j, k0);
+      }
+    }
+  }
+  return r;
+}
+// slabs holding cell (i, j, k) of a computed row: the column's x / y slabs
+// and the plane's z slabs
+__device__ __forceinline__ int face_mask(const StageArgs& g, const AuxPre& a, int i, int j, int k) {
+  return a.mask | (in_face(g.face[4], i, j, k) ? 16 : 0) | (in_face(g.face[5], i, j, k) ? 32 : 0);
+}
+template <int KIND, bool FULL>
+__device__ __forceinline__ void aux_load(const AuxPre& a, int k, int k0, int setin, AuxVals& r) {
+  const float* b = a.base + (long)(k - k0) * a.kst;
+  const long c = a.cnt;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    r.p[m] = b[setin * 6 * c + m * c];
+    r.a[m] = FULL && KIND == 1 ? b[m * c] : 0.f;
+    r.t[m] = FULL && KIND != 0 ? b[18 * c + m * c] : 0.f;
+  }
+}
+// velocity half: psi, set A and set T ahead; stress half (more live RK
+// operands): psi only
+constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = false;
 
 // CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
 // reference's face order x1 x2 y1 y2 z1 z2 (mod_abs_npml.F90:1149-2123, the
@@ -129,15 +191,18 @@ template <int KIND>
 __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k, float h[3], float rr, float DxTxx,
                                         float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz, float DzTzz,
                                         float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz, float Txz,
-                                        float Tyz, float rzk, float rxs, float xc) {
+                                        float Tyz, float rzk, float rxs, float xc, int mask, int pf, const AuxVals& pv) {
+#pragma nounroll
   for (int n = 0; n < 6; ++n) {
     const FaceArgs& f = g.face[n];
-    if (!in_face(f, i, j, k)) continue;
+    if (!((mask >> n) & 1)) continue;
     const int ax = n >> 1;
     const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
     const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
     const long li = face_index(f, i, j, k);
-    const float p3 = aux_in(f, g, li, 3), p4 = aux_in(f, g, li, 4), p5 = aux_in(f, g, li, 5);
+    const bool pre = n == pf;
+    const float p3 = pre ? pv.p[0] : aux_in(f, g, li, 3), p4 = pre ? pv.p[1] : aux_in(f, g, li, 4),
+                p5 = pre ? pv.p[2] : aux_in(f, g, li, 5);
     float D0, D1, D2;  // corrected derivatives feeding hVx, hVy, hVz
     float q0, q1, q2;  // their psi
     int v0, v1, v2;
@@ -157,9 +222,9 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k,
     h[0] += (b1_1 * D0 - b1 * q0) * rr;
     h[1] += (b1_1 * D1 - b1 * q1) * rr;
     h[2] += (b1_1 * D2 - b1 * q2) * rr;
-    aux_rk<KIND>(f, g, li, v0, q0, Dd * D0 - ad * q0);
-    aux_rk<KIND>(f, g, li, v1, q1, Dd * D1 - ad * q1);
-    aux_rk<KIND>(f, g, li, v2, q2, Dd * D2 - ad * q2);
+    aux_rk<KIND, AUX_FULL_VEL>(f, g, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a), sel3(v0 - 3, pv.t));
+    aux_rk<KIND, AUX_FULL_VEL>(f, g, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a), sel3(v1 - 3, pv.t));
+    aux_rk<KIND, AUX_FULL_VEL>(f, g, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a), sel3(v2 - 3, pv.t));
   }
 }
 
@@ -170,18 +235,21 @@ template <int KIND>
 __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k, float h[6], float lam, float miu,
                                         float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
                                         float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
-                                        float rxs, float xc) {
+                                        float rxs, float xc, int mask, int pf, const AuxVals& pv) {
   const float tc13 = lam, tc66 = miu, tc11 = tc13 + 2.0f * tc66;
   const float tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11, tc44 = tc66, tc55 = tc66;
   const bool surf = g.freenode && k == g.nk2;
+#pragma nounroll
   for (int n = 0; n < 6; ++n) {
     const FaceArgs& f = g.face[n];
-    if (!in_face(f, i, j, k)) continue;
+    if (!((mask >> n) & 1)) continue;
     const int ax = n >> 1;
     const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
     const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
     const long li = face_index(f, i, j, k);
-    const float p0 = aux_in(f, g, li, 0), p1 = aux_in(f, g, li, 1), p2 = aux_in(f, g, li, 2);
+    const bool pre = n == pf;
+    const float p0 = pre ? pv.p[0] : aux_in(f, g, li, 0), p1 = pre ? pv.p[1] : aux_in(f, g, li, 1),
+                p2 = pre ? pv.p[2] : aux_in(f, g, li, 2);
     if (ax == 0) {
       const float D0 = (DxVx + Vz) * rzk, D1 = DxVy * rzk, D2 = (DxVz - Vx) * rzk;
       h[0] += b1_1 * tc11 * D0 - b1 * tc11 * p0;
@@ -193,9 +261,9 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
         h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
         h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
       }
-      aux_rk<KIND>(f, g, li, 0, p0, Dd * D0 - ad * p0);
-      aux_rk<KIND>(f, g, li, 1, p1, Dd * D1 - ad * p1);
-      aux_rk<KIND>(f, g, li, 2, p2, Dd * D2 - ad * p2);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else if (ax == 1) {
       const float D0 = (DyVx * rxs - Vy * xc) * rzk;
       const float D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
@@ -209,18 +277,18 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
         h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
         h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
       }
-      aux_rk<KIND>(f, g, li, 0, p0, Dd * D0 - ad * p0);
-      aux_rk<KIND>(f, g, li, 1, p1, Dd * D1 - ad * p1);
-      aux_rk<KIND>(f, g, li, 2, p2, Dd * D2 - ad * p2);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else {
       h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
       h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
       h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
       h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
       h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
-      aux_rk<KIND>(f, g, li, 0, p0, Dd * DzVx - ad * p0);
-      aux_rk<KIND>(f, g, li, 1, p1, Dd * DzVy - ad * p1);
-      aux_rk<KIND>(f, g, li, 2, p2, Dd * DzVz - ad * p2);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
     }
   }
 }
@@ -382,6 +450,9 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
     cTyy[q] = ok[q] ? pTyy[c0] : 0.f;
     cTxy[q] = ok[q] ? pTxy[c0] : 0.f;
   }
+  AuxPre pre[RY];
+#pragma unroll
+  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + BYT * q, k0, 3, comp[q]) : AuxPre{nullptr, -1, 0, 0};
   unsigned kz = sz * (unsigned)k0;
   float hxTxx = ld(pTxx, hx, kz), hxTxy = ld(pTxy, hx, kz), hxTxz = ld(pTxz, hx, kz);
   float hyTxy = ld(pTxy, hy, kz), hyTyy = ld(pTyy, hy, kz), hyTyz = ld(pTyz, hy, kz);
@@ -448,6 +519,12 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
       nhxTxx = ld(pTxx, hx, kn); nhxTxy = ld(pTxy, hx, kn); nhxTxz = ld(pTxz, hx, kn);
       nhyTxy = ld(pTxy, hy, kn); nhyTyy = ld(pTyy, hy, kn); nhyTyz = ld(pTyz, hy, kn);
     }
+    AuxVals pv[RY];
+    if constexpr (PML) {
+#pragma unroll
+      for (int q = 0; q < RY; ++q)
+        if (pre[q].face >= 0) aux_load<KIND, AUX_FULL_VEL>(pre[q], k, k0, g.setin, pv[q]);
+    }
     __syncthreads();
     const float ze = g.zeta_z[k], rzk = g.rz[k];
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
@@ -477,7 +554,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
         h[2] = rr[q] * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
         if constexpr (PML)
           if (pml_here) pml_vel<KIND>(g, i, j0 + ty + BYT * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
-                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc);
+                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + BYT * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
@@ -564,6 +641,9 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
       w[q][2][s] = ok[q] ? pV2[c0 + o * (int)sz] : 0.f;
     }
   }
+  AuxPre pre[RY];
+#pragma unroll
+  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + BYT * q, k0, 0, comp[q]) : AuxPre{nullptr, -1, 0, 0};
   unsigned kz = sz * (unsigned)k0;
   float hx0 = ld(pV0, hx, kz), hx1 = ld(pV1, hx, kz), hx2 = ld(pV2, hx, kz);
   float hy0 = ld(pV0, hy, kz), hy1 = ld(pV1, hy, kz), hy2 = ld(pV2, hy, kz);
@@ -622,6 +702,12 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
       nx0 = ld(pV0, hx, kn); nx1 = ld(pV1, hx, kn); nx2 = ld(pV2, hx, kn);
       ny0 = ld(pV0, hy, kn); ny1 = ld(pV1, hy, kn); ny2 = ld(pV2, hy, kn);
     }
+    AuxVals pv[RY];
+    if constexpr (PML) {
+#pragma unroll
+      for (int q = 0; q < RY; ++q)
+        if (pre[q].face >= 0) aux_load<KIND, AUX_FULL_STR>(pre[q], k, k0, g.setin, pv[q]);
+    }
     __syncthreads();
     const float ze = g.zeta_z[k], rzk = g.rz[k];
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
@@ -665,7 +751,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
         h[5] = 2.0f * miu * E23;
         if constexpr (PML)
           if (pml_here) pml_str<KIND>(g, i, j0 + ty + BYT * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
-                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc);
+                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + BYT * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
           const long e = (TXX + f) * fs + c;
