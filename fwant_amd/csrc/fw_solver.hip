@@ -411,7 +411,7 @@ struct fw_solver_s {
   int* d_fluidcol = nullptr;  // per (i, j) column: 1 if any cell has mu <= SEIS_ZERO
   int tilecfg = 0;            // stage-kernel tile shape index
   int4* d_items = nullptr;    // stage work items: PML-shell items, then interior items
-  int n_lean = 0, n_pml = 0;
+  int n_narrow = 0, n_lean = 0, n_pml = 0;
   // host copies
   std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
   std::vector<float> hlam, hmu, hrho;
@@ -1112,40 +1112,52 @@ static StageFn str_table(int c, int p, int kind) {
   return T[c][p][kind];
 }
 
-// Work items of the stage kernels: TX x TY tiles over chunks of k planes,
-// split into items that touch a CFS-PML slab (PML body) and interior items
-// (lean body).  Stored PML items first; each part ordered by chunk, tile
-// row, tile column.
+// Work items of the stage kernels: tiles over chunks of k planes in three
+// parts: narrow (TXN-column) tiles covering the remainder of the x extent
+// after whole TX-column tiles when it is at most 32 columns (narrow body,
+// with the CFS-PML corrections), wide tiles that touch a CFS-PML slab (PML
+// body), wide interior tiles (lean body).  Each part is ordered by chunk,
+// tile row, tile column.
 static int build_items(fw_handle h) {
   const int TY = TILE_BYT[h->tilecfg] * TILE_RY[h->tilecfg];
+  const int TYN = TY * (TX / TXN);
   const PmlFace* F = h->face;
-  // Tiles from ni1 / nj1 (the last one of a row or column partial).  A tile
-  // that touches an x or y slab runs the PML body over its whole depth;
-  // cutting tiles at the slab boundaries instead costs an extra (mostly
-  // idle) tile per row, which measures slower than running 51 interior
-  // columns through the PML body.
+  // A wide tile that touches an x or y slab runs the PML body over its whole
+  // depth; cutting tiles at the slab boundaries instead costs an extra
+  // (mostly idle) tile per row, which measures slower than running 51
+  // interior columns through the PML body.
   struct Tile { int lo, n; bool pml; };
-  auto tiles = [](int lo, int hi, int w, const PmlFace& f1, const PmlFace& f2, int ax) {
+  auto tile = [](int a, int n, const PmlFace& f1, const PmlFace& f2, int ax) {
+    return Tile{a, n, (f1.on && a <= f1.hi[ax]) || (f2.on && a + n - 1 >= f2.lo[ax])};
+  };
+  auto tiles = [&](int lo, int hi, int w, const PmlFace& f1, const PmlFace& f2, int ax) {
     std::vector<Tile> v;
-    for (int a = lo; a <= hi; a += w) {
-      const int n = std::min(w, hi - a + 1);
-      v.push_back({a, n, (f1.on && a <= f1.hi[ax]) || (f2.on && a + n - 1 >= f2.lo[ax])});
-    }
+    for (int a = lo; a <= hi; a += w) v.push_back(tile(a, std::min(w, hi - a + 1), f1, f2, ax));
     return v;
   };
-  const std::vector<Tile> xt = tiles(h->ni1, h->ni2, TX, F[0], F[1], 0);
+  const int ni = h->ni2 - h->ni1 + 1, rem = ni % TX;
+  const int nwide = rem > 0 && rem <= 32 && ni > TX ? ni / TX : (ni + TX - 1) / TX;
+  std::vector<Tile> xt, xn;  // wide, narrow
+  for (int t = 0; t < nwide; ++t) {
+    const int a = h->ni1 + t * TX;
+    xt.push_back(tile(a, std::min(TX, h->ni2 - a + 1), F[0], F[1], 0));
+  }
+  for (int a = h->ni1 + nwide * TX; a <= h->ni2; a += TXN) xn.push_back(tile(a, std::min(TXN, h->ni2 - a + 1), F[0], F[1], 0));
   const std::vector<Tile> yt = tiles(h->nj1, h->nj2, TY, F[2], F[3], 1);
+  const std::vector<Tile> ytn = tiles(h->nj1, h->nj2, TYN, F[2], F[3], 1);
   const int zlo = F[4].on ? F[4].hi[2] : h->nk1 - 1;  // last plane of the z1 slab
   const int zhi = F[5].on ? F[5].lo[2] : h->nk2 + 1;  // first plane of the z2 slab
   const int mid = std::max(0, zhi - zlo - 1);
-  const int ntile = (int)(xt.size() * yt.size());
+  const int ntile = (int)(xt.size() * yt.size() + xn.size() * ytn.size());
   int nch = std::max(1, std::min(std::max(1, mid / 16), (2048 + ntile - 1) / ntile));
   const int cs = std::max(1, (std::max(mid, 1) + nch - 1) / nch);
-  std::vector<int4> lean, pml;
+  std::vector<int4> narrow, lean, pml;
   auto add_range = [&](std::vector<int4>& v, const Tile& x, const Tile& y, int ka, int kb) {
     for (int k0 = ka; k0 <= kb; k0 += cs)
       v.push_back(make_int4(x.lo | (x.n << 16), y.lo | (y.n << 16), k0, std::min(k0 + cs - 1, kb)));
   };
+  for (const Tile& y : ytn)
+    for (const Tile& x : xn) add_range(narrow, x, y, h->nk1, h->nk2);
   for (const Tile& y : yt)
     for (const Tile& x : xt) {
       if (x.pml || y.pml) {
@@ -1161,13 +1173,16 @@ static int build_items(fw_handle h) {
     if ((a.y & 0xffff) != (b.y & 0xffff)) return (a.y & 0xffff) < (b.y & 0xffff);
     return (a.x & 0xffff) < (b.x & 0xffff);
   };
+  std::stable_sort(narrow.begin(), narrow.end(), order);
   std::stable_sort(lean.begin(), lean.end(), order);
   std::stable_sort(pml.begin(), pml.end(), order);
   hipFree(h->d_items);
   h->d_items = nullptr;
+  h->n_narrow = (int)narrow.size();
   h->n_lean = (int)lean.size();
   h->n_pml = (int)pml.size();
-  std::vector<int4> all(pml);
+  std::vector<int4> all(narrow);
+  all.insert(all.end(), pml.begin(), pml.end());
   all.insert(all.end(), lean.begin(), lean.end());
   if (!all.empty()) {
     HIPCHK(hipMalloc(&h->d_items, sizeof(int4) * all.size()));
@@ -1233,8 +1248,9 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   const dim3 sblk(TX, TILE_BYT[tc]);
   if (h->timing) hipEventRecord(h->ev[2 * h->nstage], h->stream);
   g.items = h->d_items;
-  g.nitems = h->n_pml + h->n_lean;
-  g.npml = h->n_pml;
+  g.nitems = h->n_narrow + h->n_pml + h->n_lean;
+  g.p1 = h->n_narrow;
+  g.p2 = h->n_narrow + h->n_pml;
   g.bstride = h->bstride;
   const int W = h->width > 0 ? h->width : nact;
   for (int s0 = 0; s0 < nact && g.nitems > 0; s0 += W) {  // members s0 .. s0+W-1 in one launch

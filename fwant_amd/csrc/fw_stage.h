@@ -8,21 +8,27 @@
 // the two kernels.
 //
 // Work decomposition (both kernels): a 256-thread block (4 waves) owns a
-// TX x TY tile of (i, j) columns over a chunk of k planes and marches k.
+// TW x TY tile of (i, j) columns over a chunk of k planes and marches k.
+// Tiles are TW = 64 columns wide (a wave row is one 256-byte row segment),
+// except the narrow tiles at the x edge of the volume (the remainder of the
+// x extent after whole 64-column tiles), which are TW = 16 columns wide: a
+// wave then covers 4 rows of 16 columns, and no lanes idle on a 16-column
+// remainder.
 // The current k plane of every field that needs an x or y derivative is
 // staged in LDS with its one-sided halo (F: -1..+3, B: -3..+1), double
 // buffered so one barrier per plane suffices; z derivatives come from a
 // 5-slot register window per column.  While plane k is computed, the loads
 // of plane k+1 (window lead, halo) and the RK operands of plane k are in
-// flight.  Wave w owns rows j0 + w + 4q (q < RY): every global access of a
-// wave is 64 consecutive floats of one row.  Offsets are 32-bit relative to
-// per-field base pointers.
+// flight.  In a 64-wide tile wave w owns rows j0 + w + 4q (q < RY): every
+// global access of a wave is 64 consecutive floats of one row.  Offsets are
+// 32-bit relative to per-field base pointers.
 #pragma once
 #include "fw_common.h"
 
 namespace fw {
 
-constexpr int TX = 64;
+constexpr int TX = 64;   // lanes per wave, columns of a wide tile
+constexpr int TXN = 16;  // columns of a narrow tile
 
 // Streamed operands (RK registers m and t, the stage output, the media) are
 // touched once per stage: non-temporal accesses keep them from evicting the
@@ -76,7 +82,7 @@ struct StageArgs {
   int ni1, ni2, nj1, nj2, nk1, nk2;
   int freenode;
   const int4* __restrict__ items;  // work items (i0 | ncol << 16, j0 | nrow << 16, k0, k1): a tile over a chunk of planes
-  int nitems, npml;
+  int nitems, p1, p2;  // item parts: narrow [0, p1), wide PML [p1, p2), wide interior
   float a, b;
   // CFS-PML: per-axis profiles 1/B, A + D/B, D/B (x: nx, y: ny, z: nz entries)
   const float* __restrict__ pb1;
@@ -110,7 +116,7 @@ __device__ __forceinline__ float* aux_base(const FaceArgs& f, const StageArgs& g
 struct AuxVals {
   float p[3], a[3], t[3];
 };
-__device__ __forceinline__ float sel3(int m, const float v[3]) { return m == 0 ? v[0] : (m == 1 ? v[1] : v[2]); }
+__device__ __forceinline__ float sel3(int m, float v0, float v1, float v2) { return m == 0 ? v0 : (m == 1 ? v1 : v2); }
 
 // auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105);
 // pre: the set-A / set-T values were loaded ahead (pa, pt)
@@ -179,19 +185,19 @@ __device__ __forceinline__ void aux_load(const AuxPre& a, int k, int k0, int set
     r.t[m] = FULL && KIND != 0 ? b[18 * c + m * c] : 0.f;
   }
 }
-// velocity half: psi, set A and set T ahead; stress half (more live RK
-// operands): psi only
+// velocity half of the wide tiles: psi, set A and set T ahead; narrow tiles
+// and the stress half (more live RK operands): psi only
 constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = false;
 
 // CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
 // reference's face order x1 x2 y1 y2 z1 z2 (mod_abs_npml.F90:1149-2123, the
 // stress-derivative half).  D* are the raw one-sided derivatives times the
 // 1-D metric (before fluid zeroing).
-template <int KIND>
+template <int KIND, bool FULL>
 __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k, float h[3], float rr, float DxTxx,
                                         float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz, float DzTzz,
                                         float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz, float Txz,
-                                        float Tyz, float rzk, float rxs, float xc, int mask, int pf, const AuxVals& pv) {
+                                        float Tyz, float rzk, float rxs, float xc, int mask, int pf, const AuxVals pv) {
 #pragma nounroll
   for (int n = 0; n < 6; ++n) {
     const FaceArgs& f = g.face[n];
@@ -222,9 +228,9 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k,
     h[0] += (b1_1 * D0 - b1 * q0) * rr;
     h[1] += (b1_1 * D1 - b1 * q1) * rr;
     h[2] += (b1_1 * D2 - b1 * q2) * rr;
-    aux_rk<KIND, AUX_FULL_VEL>(f, g, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a), sel3(v0 - 3, pv.t));
-    aux_rk<KIND, AUX_FULL_VEL>(f, g, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a), sel3(v1 - 3, pv.t));
-    aux_rk<KIND, AUX_FULL_VEL>(f, g, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a), sel3(v2 - 3, pv.t));
+    aux_rk<KIND, FULL>(f, g, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<KIND, FULL>(f, g, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<KIND, FULL>(f, g, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
   }
 }
 
@@ -235,7 +241,7 @@ template <int KIND>
 __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k, float h[6], float lam, float miu,
                                         float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
                                         float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
-                                        float rxs, float xc, int mask, int pf, const AuxVals& pv) {
+                                        float rxs, float xc, int mask, int pf, const AuxVals pv) {
   const float tc13 = lam, tc66 = miu, tc11 = tc13 + 2.0f * tc66;
   const float tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11, tc44 = tc66, tc55 = tc66;
   const bool surf = g.freenode && k == g.nk2;
@@ -296,12 +302,14 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
 // Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD
 // receives a contiguous run of work items (items are ordered by chunk, then
 // tile row, then tile column, so shared halos stay in one L2).  The list
-// holds the npml PML-shell items first (dispatched first: they are the
-// heavier ones), then the interior items; each part is remapped on its own.
+// holds three parts, dispatched in this order: the narrow edge tiles
+// [0, p1), the wide PML-shell tiles [p1, p2) (the heavier ones), the wide
+// interior tiles [p2, nitems); each part is remapped on its own.
 __device__ __forceinline__ int item_index(const StageArgs& g) {
   const int b = blockIdx.x;
-  const int base = b < g.npml ? 0 : g.npml;
-  const int n = b < g.npml ? g.npml : g.nitems - g.npml;
+  const int base = b < g.p1 ? 0 : (b < g.p2 ? g.p1 : g.p2);
+  const int end = b < g.p1 ? g.p1 : (b < g.p2 ? g.p2 : g.nitems);
+  const int n = end - base;
   const int bb = b - base, q = n / 8;
   return base + ((bb < 8 * q) ? (bb % 8) * q + bb / 8 : bb);
 }
@@ -343,7 +351,7 @@ __device__ __forceinline__ float zc(const float* w) {
 
 // Halo slot of one thread in one LDS plane: x halo (XL left + XR right
 // columns of the TY rows, threads tid < 4*TY) or y halo (YL + YR rows of the
-// TX columns, every thread).  Slots outside the volume or beyond what the
+// TW columns, every thread).  Slots outside the volume or beyond what the
 // tile's computed columns ie / rows je read (partial tiles) hold zero and
 // never feed a computed cell.
 struct HaloSlot {
@@ -351,13 +359,13 @@ struct HaloSlot {
   int lds;
   bool ok;
 };
-template <int XF, int TY, int LW>
+template <int XF, int TW, int TY, int LW>
 __device__ __forceinline__ HaloSlot xhalo_slot(const StageArgs& g, int i0, int j0, int ie, int je, int tid) {
   constexpr int XL = XF ? 1 : 3;
   HaloSlot h{0u, 0, false};
   if (tid < 4 * TY) {
     const int r = tid % TY, s = tid / TY;
-    const int col = s < XL ? i0 - XL + s : i0 + TX + (s - XL);
+    const int col = s < XL ? i0 - XL + s : i0 + TW + (s - XL);
     const int row = j0 + r;
     h.ok = col < g.nx && col <= ie + 3 && row < g.ny && row <= je;
     h.off = h.ok ? (unsigned)(col + g.sy * row) : 0u;
@@ -367,12 +375,12 @@ __device__ __forceinline__ HaloSlot xhalo_slot(const StageArgs& g, int i0, int j
   }
   return h;
 }
-template <int YF, int TY, int LW>
+template <int YF, int TW, int TY, int LW>
 __device__ __forceinline__ HaloSlot yhalo_slot(const StageArgs& g, int i0, int j0, int ie, int je, int tid) {
   constexpr int YL = YF ? 1 : 3;
   HaloSlot h{0u, -1, false};
-  if (tid >= 4 * TX) return h;
-  const int c = tid % TX, s = tid / TX;
+  if (tid >= 4 * TW) return h;
+  const int c = tid % TW, s = tid / TW;
   const int row = s < YL ? j0 - YL + s : j0 + TY + (s - YL);
   const int col = i0 + c;
   h.ok = col < g.nx && col <= ie && row < g.ny && row <= je + 3;
@@ -391,15 +399,18 @@ __device__ __forceinline__ float ld(const float* __restrict__ p, const HaloSlot&
 // (:917-926), RK update of Vx Vy Vz.
 // LDS: Txx (x), Txy (x, y), Txz (x), Tyy (y), Tyz (y); z window: Tzz Txz Tyz.
 // ---------------------------------------------------------------------------
-template <int BYT, int RY>
+template <int BYT, int RY, int TW>
 struct VelLds {
-  static constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, NL = 5, SIZE = 2 * NL * PL;
+  static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, NL = 5, SIZE = 2 * NL * PL;
 };
-template <int XF, int YF, int ZF, int KIND, int BYT, int RY, bool PML>
+template <int XF, int YF, int ZF, int KIND, int BYT, int RY, int TW, bool PML>
 __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, float* shbase) {
-  constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, NL = 5;
+  // RS: rows per row group (the stride between a thread's rows)
+  constexpr int RS = BYT * (TX / TW), TY = RS * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, NL = 5;
+  // set-A / set-T values of the CFS-PML auxiliaries loaded ahead in wide tiles
+  constexpr bool FULLV = AUX_FULL_VEL && TW == TX;
   float(*sh)[NL * PL] = reinterpret_cast<float(*)[NL * PL]>(shbase);  // Txx Txy Txz Tyy Tyz, double buffered
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * threadIdx.y, tid = threadIdx.x + TX * threadIdx.y;
   const int i0 = it.x & 0xffff, j0 = it.y & 0xffff, k0 = it.z, k1 = it.w;
   // last computed column and row: tiles end at CFS-PML slab boundaries
   const int ie = i0 + (it.x >> 16) - 1, je = j0 + (it.y >> 16) - 1;
@@ -422,8 +433,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
   const bool iok = i < g.nx && i <= ie + 3, icomp = i <= ie;
   const int ic = min(i, g.nx - 1);
   const float xi = g.xi_x[ic], rxs = g.rxsin[ic], xc = g.xcot[ic];
-  const HaloSlot hx = xhalo_slot<XF, TY, LW>(g, i0, j0, ie, je, tid);
-  const HaloSlot hy = yhalo_slot<YF, TY, LW>(g, i0, j0, ie, je, tid);
+  const HaloSlot hx = xhalo_slot<XF, TW, TY, LW>(g, i0, j0, ie, je, tid);
+  const HaloSlot hy = yhalo_slot<YF, TW, TY, LW>(g, i0, j0, ie, je, tid);
 
   float w[RY][3][5];  // Tzz Txz Tyz
   float cTxx[RY], cTyy[RY], cTxy[RY];
@@ -432,7 +443,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
   float et[RY];
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
-    const int j = j0 + ty + BYT * q;
+    const int j = j0 + ty + RS * q;
     ok[q] = iok && j < g.ny && j <= je + 3;
     comp[q] = icomp && j <= je;
     cc[q] = ok[q] ? (unsigned)(i + g.sy * j) : 0u;
@@ -452,7 +463,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
   }
   AuxPre pre[RY];
 #pragma unroll
-  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + BYT * q, k0, 3, comp[q]) : AuxPre{nullptr, -1, 0, 0};
+  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + RS * q, k0, 3, comp[q]) : AuxPre{nullptr, -1, 0, 0};
   unsigned kz = sz * (unsigned)k0;
   float hxTxx = ld(pTxx, hx, kz), hxTxy = ld(pTxy, hx, kz), hxTxz = ld(pTxz, hx, kz);
   float hyTxy = ld(pTxy, hy, kz), hyTyy = ld(pTyy, hy, kz), hyTyz = ld(pTyz, hy, kz);
@@ -466,7 +477,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
     float* sTyz = s + 4 * PL;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
-      const int o = (ty + BYT * q + 3) * LW + tx + 3;
+      const int o = (ty + RS * q + 3) * LW + tx + 3;
       sTxx[o] = cTxx[q];
       sTxy[o] = cTxy[q];
       sTyy[o] = cTyy[q];
@@ -523,7 +534,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
     if constexpr (PML) {
 #pragma unroll
       for (int q = 0; q < RY; ++q)
-        if (pre[q].face >= 0) aux_load<KIND, AUX_FULL_VEL>(pre[q], k, k0, g.setin, pv[q]);
+        if (pre[q].face >= 0) aux_load<KIND, FULLV>(pre[q], k, k0, g.setin, pv[q]);
     }
     __syncthreads();
     const float ze = g.zeta_z[k], rzk = g.rz[k];
@@ -532,7 +543,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
     for (int q = 0; q < RY; ++q) {
       if (comp[q]) {
         const unsigned c = cc[q] + kz;
-        const int o = (ty + BYT * q + 3) * LW + tx + 3;
+        const int o = (ty + RS * q + 3) * LW + tx + 3;
         const float DxTxx = dlds<XF>(sTxx + o, 1) * xi;
         float DxTxy = dlds<XF>(sTxy + o, 1) * xi;
         float DxTxz = dlds<XF>(sTxz + o, 1) * xi;
@@ -553,8 +564,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
         h[1] = rr[q] * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
         h[2] = rr[q] * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
         if constexpr (PML)
-          if (pml_here) pml_vel<KIND>(g, i, j0 + ty + BYT * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
-                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + BYT * q, k), pre[q].face, pv[q]);
+          if (pml_here) pml_vel<KIND, FULLV>(g, i, j0 + ty + RS * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
+                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
@@ -590,15 +601,15 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
 // Tzz Txz Tyz are the pre-image stresses saved by k_surface.
 // LDS: Vx Vy Vz (x and y halos); z window: Vx Vy Vz.
 // ---------------------------------------------------------------------------
-template <int BYT, int RY>
+template <int BYT, int RY, int TW>
 struct StrLds {
-  static constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH, SIZE = 2 * 3 * PL;
+  static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, SIZE = 2 * 3 * PL;
 };
-template <int XF, int YF, int ZF, int KIND, int BYT, int RY, bool PML>
+template <int XF, int YF, int ZF, int KIND, int BYT, int RY, int TW, bool PML>
 __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, float* shbase) {
-  constexpr int TY = BYT * RY, LW = TX + 6, LH = TY + 6, PL = LW * LH;
+  constexpr int RS = BYT * (TX / TW), TY = RS * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH;
   float(*sh)[3 * PL] = reinterpret_cast<float(*)[3 * PL]>(shbase);  // Vx Vy Vz, double buffered
-  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + TX * ty;
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * threadIdx.y, tid = threadIdx.x + TX * threadIdx.y;
   const int i0 = it.x & 0xffff, j0 = it.y & 0xffff, k0 = it.z, k1 = it.w;
   // last computed column and row: tiles end at CFS-PML slab boundaries
   const int ie = i0 + (it.x >> 16) - 1, je = j0 + (it.y >> 16) - 1;
@@ -618,8 +629,8 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   const bool iok = i < g.nx && i <= ie + 3, icomp = i <= ie;
   const int ic = min(i, g.nx - 1);
   const float xi = g.xi_x[ic], rxs = g.rxsin[ic], xc = g.xcot[ic];
-  const HaloSlot hx = xhalo_slot<XF, TY, LW>(g, i0, j0, ie, je, tid);
-  const HaloSlot hy = yhalo_slot<YF, TY, LW>(g, i0, j0, ie, je, tid);
+  const HaloSlot hx = xhalo_slot<XF, TW, TY, LW>(g, i0, j0, ie, je, tid);
+  const HaloSlot hy = yhalo_slot<YF, TW, TY, LW>(g, i0, j0, ie, je, tid);
 
   float w[RY][3][5];
   bool ok[RY], comp[RY];
@@ -627,7 +638,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   float et[RY];
 #pragma unroll
   for (int q = 0; q < RY; ++q) {
-    const int j = j0 + ty + BYT * q;
+    const int j = j0 + ty + RS * q;
     ok[q] = iok && j < g.ny && j <= je + 3;
     comp[q] = icomp && j <= je;
     cc[q] = ok[q] ? (unsigned)(i + g.sy * j) : 0u;
@@ -643,7 +654,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   }
   AuxPre pre[RY];
 #pragma unroll
-  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + BYT * q, k0, 0, comp[q]) : AuxPre{nullptr, -1, 0, 0};
+  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + RS * q, k0, 0, comp[q]) : AuxPre{nullptr, -1, 0, 0};
   unsigned kz = sz * (unsigned)k0;
   float hx0 = ld(pV0, hx, kz), hx1 = ld(pV1, hx, kz), hx2 = ld(pV2, hx, kz);
   float hy0 = ld(pV0, hy, kz), hy1 = ld(pV1, hy, kz), hy2 = ld(pV2, hy, kz);
@@ -652,7 +663,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
     float* s = sh[(k - k0) & 1];
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
-      const int o = (ty + BYT * q + 3) * LW + tx + 3;
+      const int o = (ty + RS * q + 3) * LW + tx + 3;
       s[o] = zc<ZF>(w[q][0]);
       s[PL + o] = zc<ZF>(w[q][1]);
       s[2 * PL + o] = zc<ZF>(w[q][2]);
@@ -715,7 +726,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
     for (int q = 0; q < RY; ++q) {
       if (comp[q]) {
         const unsigned c = cc[q] + kz;
-        const int o = (ty + BYT * q + 3) * LW + tx + 3;
+        const int o = (ty + RS * q + 3) * LW + tx + 3;
         const float DxVx = dlds<XF>(s + o, 1) * xi;
         const float DxVy = dlds<XF>(s + PL + o, 1) * xi;
         const float DxVz = dlds<XF>(s + 2 * PL + o, 1) * xi;
@@ -750,8 +761,8 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
         h[4] = 2.0f * miu * E13;
         h[5] = 2.0f * miu * E23;
         if constexpr (PML)
-          if (pml_here) pml_str<KIND>(g, i, j0 + ty + BYT * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
-                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + BYT * q, k), pre[q].face, pv[q]);
+          if (pml_here) pml_str<KIND>(g, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
+                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
           const long e = (TXX + f) * fs + c;
@@ -775,27 +786,36 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   }
 }
 
-// One launch per stage and kind: the PML-shell items run the PML body, the
-// interior items the lean body (each keeps its own register allocation).
+// One launch per stage and kind: the narrow edge tiles run the narrow
+// body (with the CFS-PML corrections), the wide PML-shell tiles the PML
+// body, the wide interior tiles the lean body.  At most 168 VGPRs (3 waves
+// per SIMD): the inner RK kind of k_vel then keeps a few values in scratch,
+// which measures faster than 2 waves per SIMD.
 template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
-__global__ __launch_bounds__(TX* BYT) void k_vel(StageArgs g) {
-  __shared__ float sh[VelLds<BYT, RY>::SIZE];
+__global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) void k_vel(StageArgs g) {
+  constexpr int SW = VelLds<BYT, RY, TX>::SIZE, SN = VelLds<BYT, RY, TXN>::SIZE;
+  __shared__ float sh[SW > SN ? SW : SN];
   const int L = item_index(g);
   const int4 it = g.items[L];
-  if (L < g.npml)
-    vel_body<XF, YF, ZF, KIND, BYT, RY, true>(g, it, sh);
+  if (L < g.p1)
+    vel_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, sh);
+  else if (L < g.p2)
+    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, sh);
   else
-    vel_body<XF, YF, ZF, KIND, BYT, RY, false>(g, it, sh);
+    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, sh);
 }
 template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
-__global__ __launch_bounds__(TX* BYT) void k_str(StageArgs g) {
-  __shared__ float sh[StrLds<BYT, RY>::SIZE];
+__global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) void k_str(StageArgs g) {
+  constexpr int SW = StrLds<BYT, RY, TX>::SIZE, SN = StrLds<BYT, RY, TXN>::SIZE;
+  __shared__ float sh[SW > SN ? SW : SN];
   const int L = item_index(g);
   const int4 it = g.items[L];
-  if (L < g.npml)
-    str_body<XF, YF, ZF, KIND, BYT, RY, true>(g, it, sh);
+  if (L < g.p1)
+    str_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, sh);
+  else if (L < g.p2)
+    str_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, sh);
   else
-    str_body<XF, YF, ZF, KIND, BYT, RY, false>(g, it, sh);
+    str_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, sh);
 }
 
 }  // namespace fw

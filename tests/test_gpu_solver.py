@@ -34,9 +34,21 @@ def interior(a, m):
     return a[3:3 + m.ni, 3:3 + m.nj, 3:3 + m.nk]
 
 
-@pytest.mark.parametrize("case", ["free_surface", "pml6", "moment_only", "water"])
-def test_solver_parity(gpu_lib, case):
-    if case == "free_surface":
+@pytest.mark.parametrize("case", ["free_surface", "pml6", "moment_only", "water", "narrow_edge", "narrow_two",
+                                  "narrow_tall"])
+def test_solver_parity(gpu_lib, case, monkeypatch):
+    # narrow_*: the x extent leaves a remainder of at most 32 columns after
+    # whole 64-column tiles, run as 16-column tiles holding the x2 CFS-PML slab
+    # (12 columns: one partial narrow tile; 32: two); narrow_tall with the
+    # 16-row tile shape (64-row narrow tiles)
+    if case == "narrow_edge":
+        m = models.small_test(ni=76, nj=40, nk=24, nt=60, seed=14)
+    elif case == "narrow_two":
+        m = models.small_test(ni=96, nj=36, nk=24, nt=60, seed=15, freenode=False)
+    elif case == "narrow_tall":
+        monkeypatch.setenv("FWANT_TILE", "1")
+        m = models.small_test(ni=76, nj=70, nk=20, nt=60, seed=16)
+    elif case == "free_surface":
         m = models.small_test(nt=96)
     elif case == "pml6":
         m = models.small_test(nt=96, freenode=False, seed=11)
@@ -45,7 +57,7 @@ def test_solver_parity(gpu_lib, case):
         m.force = models.Sources()
     else:
         m = models.small_test(nt=96, seed=13, water=True)
-    n = 96
+    n = min(96, m.nt)
     print(f"case={case} grid={m.ni}x{m.nj}x{m.nk} steps={n}", flush=True)
     s, o, g, r = run_pair(m, n)
     assert g.shape == r.shape and g.shape[0] == n
