@@ -35,16 +35,19 @@ def interior(a, m):
 
 
 @pytest.mark.parametrize("case", ["free_surface", "pml6", "moment_only", "water", "narrow_edge", "narrow_two",
-                                  "narrow_tall"])
+                                  "narrow_tall", "narrow_x1"])
 def test_solver_parity(gpu_lib, case, monkeypatch):
     # narrow_*: the x extent leaves a remainder of at most 32 columns after
     # whole 64-column tiles, run as 16-column tiles holding the x2 CFS-PML slab
     # (12 columns: one partial narrow tile; 32: two); narrow_tall with the
-    # 16-row tile shape (64-row narrow tiles)
+    # 16-row tile shape (64-row narrow tiles); narrow_x1: two wide tiles, the
+    # first holding the x1 slab, then one narrow tile, CFS-PML on 6 sides
     if case == "narrow_edge":
         m = models.small_test(ni=76, nj=40, nk=24, nt=60, seed=14)
     elif case == "narrow_two":
         m = models.small_test(ni=96, nj=36, nk=24, nt=60, seed=15, freenode=False)
+    elif case == "narrow_x1":
+        m = models.small_test(ni=140, nj=30, nk=20, nt=50, seed=17, freenode=False)
     elif case == "narrow_tall":
         monkeypatch.setenv("FWANT_TILE", "1")
         m = models.small_test(ni=76, nj=70, nk=20, nt=60, seed=16)
