@@ -1147,10 +1147,10 @@ static int build_items(fw_handle h) {
   const std::vector<Tile> ytn = tiles(h->nj1, h->nj2, TYN, F[2], F[3], 1);
   const int zlo = F[4].on ? F[4].hi[2] : h->nk1 - 1;  // last plane of the z1 slab
   const int zhi = F[5].on ? F[5].lo[2] : h->nk2 + 1;  // first plane of the z2 slab
-  const int mid = std::max(0, zhi - zlo - 1);
+  const int nk = h->nk2 - h->nk1 + 1;
   const int ntile = (int)(xt.size() * yt.size() + xn.size() * ytn.size());
-  int nch = std::max(1, std::min(std::max(1, mid / 16), (2048 + ntile - 1) / ntile));
-  const int cs = std::max(1, (std::max(mid, 1) + nch - 1) / nch);
+  int nch = std::max(1, std::min(std::max(1, nk / 16), (2048 + ntile - 1) / ntile));
+  const int cs = std::max(1, (nk + nch - 1) / nch);
   std::vector<int4> narrow, lean, pml;
   auto add_range = [&](std::vector<int4>& v, const Tile& x, const Tile& y, int ka, int kb) {
     for (int k0 = ka; k0 <= kb; k0 += cs)
@@ -1163,9 +1163,14 @@ static int build_items(fw_handle h) {
       if (x.pml || y.pml) {
         add_range(pml, x, y, h->nk1, h->nk2);
       } else {
-        if (F[4].on) add_range(pml, x, y, h->nk1, zlo);
-        if (mid > 0) add_range(lean, x, y, zlo + 1, zhi - 1);
-        if (F[5].on) add_range(pml, x, y, zhi, h->nk2);
+        // chunks over the whole depth; one that holds planes of a z slab runs
+        // the PML body (a short slab-only chunk would pay the window start
+        // of a whole chunk)
+        for (int k0 = h->nk1; k0 <= h->nk2; k0 += cs) {
+          const int k1 = std::min(k0 + cs - 1, h->nk2);
+          const int4 it = make_int4(x.lo | (x.n << 16), y.lo | (y.n << 16), k0, k1);
+          (k0 <= zlo || k1 >= zhi ? pml : lean).push_back(it);
+        }
       }
     }
   auto order = [](const int4& a, const int4& b) {
