@@ -23,19 +23,21 @@ SNAP = dict(subs=(10, 10, 20), subc=(20, 18, 10), subt=(1, 1, 1), tinv=4, fields
             max_records=20)
 
 
-def base():
-    return models.small_test(ni=40, nj=36, nk=30, nt=64, seed=51, moment=False)
+def base(ni=40):
+    return models.small_test(ni=ni, nj=36, nk=30, nt=64, seed=51, moment=False)
 
 
-def single(member):
-    m, b = models.virtual_sources(base(), [STATIONS[member]], [NSTEPS[member]], f0=0.6)
+def single(member, ni=40):
+    m, b = models.virtual_sources(base(ni), [STATIONS[member]], [NSTEPS[member]], f0=0.6)
     return m
 
 
-def test_members_equal_single_solves(gpu_lib):
+# ni = 76: one wide tile and a 12-column narrow edge tile holding the x2 slab
+@pytest.mark.parametrize("ni", [40, 76])
+def test_members_equal_single_solves(gpu_lib, ni):
     from fwant_amd.solver import Solver
-    m, b = models.virtual_sources(base(), STATIONS, NSTEPS, f0=0.6)
-    print(f"batch of {len(STATIONS)} members, record lengths {NSTEPS}", flush=True)
+    m, b = models.virtual_sources(base(ni), STATIONS, NSTEPS, f0=0.6)
+    print(f"ni={ni}: batch of {len(STATIONS)} members, record lengths {NSTEPS}", flush=True)
     snaps = [dict(SNAP), dict(SNAP, format="bf16")]
     s = Solver(m, batch=b, snapshots=snaps, max_records=80)
     s.run(max(NSTEPS))
@@ -43,7 +45,7 @@ def test_members_equal_single_solves(gpu_lib):
         s.select(q)
         rb = s.receivers()
         assert rb.shape[0] == NSTEPS[q]
-        one = Solver(single(q), snapshots=snaps, max_records=80)
+        one = Solver(single(q, ni), snapshots=snaps, max_records=80)
         one.run(NSTEPS[q])
         np.testing.assert_array_equal(rb, one.receivers())
         np.testing.assert_array_equal(s.snapshot(0), one.snapshot(0))
