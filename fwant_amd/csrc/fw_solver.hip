@@ -256,6 +256,21 @@ __global__ void k_setclk(int* clk, ClockSet c) {
   if (q < c.n) clk[q] = c.v[q];
 }
 
+// One volume between the dense host order (row pitch nx) and the pitched
+// device layout (row pitch px), through a dense device staging copy: the
+// host transfer stays one contiguous copy (a 2-D copy from pageable host
+// memory moves one row at a time).
+__global__ __launch_bounds__(256) void k_repitch(float* __restrict__ dst, const float* __restrict__ src, int nx, long px,
+                                                 long n, int to_pitched) {
+  const long q = (long)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  const long r = q / nx, i = q - r * nx;
+  if (to_pitched)
+    dst[r * px + i] = src[q];
+  else
+    dst[q] = src[r * px + i];
+}
+
 // snapshot: out[nf][npts] for the fields in the mask list
 struct SnapArgs {
   const float* __restrict__ W;
@@ -409,6 +424,7 @@ struct fw_solver_s {
   float *xi_x, *eta_y, *zeta_z, *rz, *rxsin, *xcot;
   float* d_msurf = nullptr;
   int* d_fluidcol = nullptr;  // per (i, j) column: 1 if any cell has mu <= SEIS_ZERO
+  float* d_stage = nullptr;   // dense staging copy of one volume (host transfers)
   int tilecfg = 0;            // stage-kernel tile shape index
   int4* d_items = nullptr;    // stage work items: PML-shell items, then interior items
   int n_narrow = 0, n_lean = 0, n_pml = 0;
@@ -528,6 +544,7 @@ int fw_destroy(fw_handle h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   hipFree(h->d_fluidcol);
   hipFree(h->d_sets); hipFree(h->d_media); hipFree(h->d_1d); hipFree(h->d_msurf); hipFree(h->d_pml);
+  hipFree(h->d_stage);
   for (auto& f : h->face) hipFree(f.d);
   for (void* p : h->d_src_allocs) hipFree(p);
   hipFree(h->d_pt); hipFree(h->d_rec);
@@ -625,13 +642,22 @@ int fw_build_metric(fw_handle h, const double* x, const double* y, const double*
 }
 
 static int upload_volume(fw_handle h, float* dst, const float* src) {
-  HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * h->px, src, sizeof(float) * h->nx, sizeof(float) * h->nx,
-                          (size_t)h->ny * h->nz, hipMemcpyHostToDevice, h->stream));
+  const long n = (long)h->nx * h->ny * h->nz;
+  if (!h->d_stage) HIPCHK(hipMalloc(&h->d_stage, sizeof(float) * n));
+  HIPCHK(hipMemcpyAsync(h->d_stage, src, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_repitch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, dst, h->d_stage, h->nx,
+                     h->px, n, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
 static int download_volume(fw_handle h, float* dst, const float* src) {
-  HIPCHK(hipMemcpy2DAsync(dst, sizeof(float) * h->nx, src, sizeof(float) * h->px, sizeof(float) * h->nx,
-                          (size_t)h->ny * h->nz, hipMemcpyDeviceToHost, h->stream));
+  const long n = (long)h->nx * h->ny * h->nz;
+  if (!h->d_stage) HIPCHK(hipMalloc(&h->d_stage, sizeof(float) * n));
+  hipLaunchKernelGGL(k_repitch, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_stage, src, h->nx,
+                     h->px, n, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(dst, h->d_stage, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
