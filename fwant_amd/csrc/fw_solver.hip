@@ -1286,7 +1286,8 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   const int W = h->width > 0 ? h->width : nact;
   for (int s0 = 0; s0 < nact && g.nitems > 0; s0 += W) {  // members s0 .. s0+W-1 in one launch
     g.slot0 = s0;
-    const dim3 grid(g.nitems, std::min(W, nact - s0));
+    g.nm = std::min(W, nact - s0);
+    const dim3 grid(g.nitems * g.nm);
     hipLaunchKernelGGL(vel_table(tc, pat, kind), grid, sblk, 0, h->stream, g);
     hipLaunchKernelGGL(str_table(tc, pat, kind), grid, sblk, 0, h->stream, g);
   }

@@ -90,14 +90,34 @@ struct StageArgs {
   const float* __restrict__ pdd;
   int n1, setin, setout, pml;
   FaceArgs face[6];
-  // batched virtual sources: member slot0 + blockIdx.y; its register sets
-  // start bstride floats after the previous member's, its surface copy
-  // 3*ps after, its PML auxiliaries 24*cnt after (per face)
+  // batched virtual sources: nm members from slot0 in one launch; a
+  // member's register sets start bstride floats after the previous
+  // member's, its surface copy 3*ps after, its PML auxiliaries 24*cnt after
+  // (per face)
   long bstride;
-  int slot0;
+  int slot0, nm;
 };
 
-__device__ __forceinline__ int member_slot(const StageArgs& g) { return g.slot0 + (int)blockIdx.y; }
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD
+// receives a contiguous run of (work item, member) pairs, item-major: the
+// members of one item run side by side on one XCD and share its media reads
+// in L2, and items are ordered by chunk, then tile row, then tile column, so
+// shared halos stay in one L2 too.  The item list holds three parts,
+// dispatched in this order: the narrow edge tiles [0, p1), the wide
+// PML-shell tiles [p1, p2) (the heavier ones), the wide interior tiles
+// [p2, nitems); each part is remapped on its own.
+struct ItemPos {
+  int item, member;
+};
+__device__ __forceinline__ ItemPos item_pos(const StageArgs& g) {
+  const int nm = g.nm, b = blockIdx.x;
+  const int e1 = g.p1 * nm, e2 = g.p2 * nm, e3 = g.nitems * nm;
+  const int base = b < e1 ? 0 : (b < e2 ? e1 : e2);
+  const int end = b < e1 ? e1 : (b < e2 ? e2 : e3);
+  const int n = end - base, bb = b - base, q = n / 8;
+  const int L = (bb < 8 * q) ? (bb % 8) * q + bb / 8 : bb;
+  return ItemPos{base / nm + L / nm, L % nm};
+}
 
 __device__ __forceinline__ long face_index(const FaceArgs& f, int i, int j, int k) {
   // a face holds fewer than 2^31 cells (checked at set-up)
@@ -106,8 +126,9 @@ __device__ __forceinline__ long face_index(const FaceArgs& f, int i, int j, int 
 __device__ __forceinline__ bool in_face(const FaceArgs& f, int i, int j, int k) {
   return f.on && i >= f.lo0 && i <= f.hi0 && j >= f.lo1 && j <= f.hi1 && k >= f.lo2 && k <= f.hi2;
 }
-__device__ __forceinline__ float* aux_base(const FaceArgs& f, const StageArgs& g) {
-  return f.d + (long)member_slot(g) * 24 * f.cnt;
+// slot: the member's slot in the batch
+__device__ __forceinline__ float* aux_base(const FaceArgs& f, int slot) {
+  return f.d + (long)slot * 24 * f.cnt;
 }
 
 // Auxiliary values of three variables of one face cell, loaded ahead of
@@ -121,10 +142,10 @@ __device__ __forceinline__ float sel3(int m, float v0, float v1, float v2) { ret
 // auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105);
 // pre: the set-A / set-T values were loaded ahead (pa, pt)
 template <int KIND, bool FULL>
-__device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, long li, int v, float psi, float dpsi,
+__device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, int slot, long li, int v, float psi, float dpsi,
                                        bool pre, float pa, float pt) {
   const long S = 6 * f.cnt, q = v * f.cnt + li;
-  float* const d = aux_base(f, g);
+  float* const d = aux_base(f, slot);
   if constexpr (KIND == 0) {
     d[g.setout * S + q] = psi + g.a * dpsi;
     d[3 * S + q] = psi + g.b * dpsi;
@@ -137,8 +158,8 @@ __device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, lo
     d[g.setout * S + q] = T + g.b * dpsi;
   }
 }
-__device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, long li, int v) {
-  return aux_base(f, g)[g.setin * 6 * f.cnt + v * f.cnt + li];
+__device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, int slot, long li, int v) {
+  return aux_base(f, slot)[g.setin * 6 * f.cnt + v * f.cnt + li];
 }
 
 // Prefetch of the auxiliary values of the first x or y face slab holding a
@@ -153,7 +174,7 @@ struct AuxPre {
   int cnt, kst;       // face cell count, index step per plane
   int mask;           // bit n: the cell is in x or y slab n (0..3)
 };
-__device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int i, int j, int k0, int v0, bool comp) {
+__device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int slot, int i, int j, int k0, int v0, bool comp) {
   AuxPre r{nullptr, -1, 0, 0, 0};
   for (int n = 0; n < 4; ++n) {
     const FaceArgs& f = g.face[n];
@@ -163,7 +184,7 @@ __device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int i, int j, int 
         r.face = n;
         r.cnt = (int)f.cnt;
         r.kst = f.ex0 * f.ex1;
-        r.base = aux_base(f, g) + (long)v0 * f.cnt + face_index(f, i, j, k0);
+        r.base = aux_base(f, slot) + (long)v0 * f.cnt + face_index(f, i, j, k0);
       }
     }
   }
@@ -194,7 +215,7 @@ constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = false;
 // stress-derivative half).  D* are the raw one-sided derivatives times the
 // 1-D metric (before fluid zeroing).
 template <int KIND, bool FULL>
-__device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k, float h[3], float rr, float DxTxx,
+__device__ __forceinline__ void pml_vel(const StageArgs& g, int slot, int i, int j, int k, float h[3], float rr, float DxTxx,
                                         float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz, float DzTzz,
                                         float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz, float Txz,
                                         float Tyz, float rzk, float rxs, float xc, int mask, int pf, const AuxVals pv) {
@@ -207,8 +228,8 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k,
     const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
     const long li = face_index(f, i, j, k);
     const bool pre = n == pf;
-    const float p3 = pre ? pv.p[0] : aux_in(f, g, li, 3), p4 = pre ? pv.p[1] : aux_in(f, g, li, 4),
-                p5 = pre ? pv.p[2] : aux_in(f, g, li, 5);
+    const float p3 = pre ? pv.p[0] : aux_in(f, g, slot, li, 3), p4 = pre ? pv.p[1] : aux_in(f, g, slot, li, 4),
+                p5 = pre ? pv.p[2] : aux_in(f, g, slot, li, 5);
     float D0, D1, D2;  // corrected derivatives feeding hVx, hVy, hVz
     float q0, q1, q2;  // their psi
     int v0, v1, v2;
@@ -228,9 +249,9 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k,
     h[0] += (b1_1 * D0 - b1 * q0) * rr;
     h[1] += (b1_1 * D1 - b1 * q1) * rr;
     h[2] += (b1_1 * D2 - b1 * q2) * rr;
-    aux_rk<KIND, FULL>(f, g, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
-    aux_rk<KIND, FULL>(f, g, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
-    aux_rk<KIND, FULL>(f, g, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<KIND, FULL>(f, g, slot, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<KIND, FULL>(f, g, slot, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<KIND, FULL>(f, g, slot, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
   }
 }
 
@@ -238,7 +259,7 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int i, int j, int k,
 // free-surface Hooke fix-ups of the x and y faces (mod_abs_npml.F90:1303-1333).
 // h: Txx Tyy Tzz Txy Txz Tyz.  DzV* are the standard 5-tap z derivatives.
 template <int KIND>
-__device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k, float h[6], float lam, float miu,
+__device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int j, int k, float h[6], float lam, float miu,
                                         float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
                                         float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
                                         float rxs, float xc, int mask, int pf, const AuxVals pv) {
@@ -254,8 +275,8 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
     const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
     const long li = face_index(f, i, j, k);
     const bool pre = n == pf;
-    const float p0 = pre ? pv.p[0] : aux_in(f, g, li, 0), p1 = pre ? pv.p[1] : aux_in(f, g, li, 1),
-                p2 = pre ? pv.p[2] : aux_in(f, g, li, 2);
+    const float p0 = pre ? pv.p[0] : aux_in(f, g, slot, li, 0), p1 = pre ? pv.p[1] : aux_in(f, g, slot, li, 1),
+                p2 = pre ? pv.p[2] : aux_in(f, g, slot, li, 2);
     if (ax == 0) {
       const float D0 = (DxVx + Vz) * rzk, D1 = DxVy * rzk, D2 = (DxVz - Vx) * rzk;
       h[0] += b1_1 * tc11 * D0 - b1 * tc11 * p0;
@@ -267,9 +288,9 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
         h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
         h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
       }
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else if (ax == 1) {
       const float D0 = (DyVx * rxs - Vy * xc) * rzk;
       const float D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
@@ -283,35 +304,20 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int i, int j, int k,
         h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
         h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
       }
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else {
       h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
       h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
       h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
       h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
       h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
     }
   }
-}
-
-// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD
-// receives a contiguous run of work items (items are ordered by chunk, then
-// tile row, then tile column, so shared halos stay in one L2).  The list
-// holds three parts, dispatched in this order: the narrow edge tiles
-// [0, p1), the wide PML-shell tiles [p1, p2) (the heavier ones), the wide
-// interior tiles [p2, nitems); each part is remapped on its own.
-__device__ __forceinline__ int item_index(const StageArgs& g) {
-  const int b = blockIdx.x;
-  const int base = b < g.p1 ? 0 : (b < g.p2 ? g.p1 : g.p2);
-  const int end = b < g.p1 ? g.p1 : (b < g.p2 ? g.p2 : g.nitems);
-  const int n = end - base;
-  const int bb = b - base, q = n / 8;
-  return base + ((bb < 8 * q) ? (bb % 8) * q + bb / 8 : bb);
 }
 
 // one-sided 5-tap operator on LDS: p points at the centre, s is the stride
@@ -404,7 +410,7 @@ struct VelLds {
   static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, NL = 5, SIZE = 2 * NL * PL;
 };
 template <int XF, int YF, int ZF, int KIND, int BYT, int RY, int TW, bool PML>
-__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, float* shbase) {
+__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int slot, float* shbase) {
   // RS: rows per row group (the stride between a thread's rows)
   constexpr int RS = BYT * (TX / TW), TY = RS * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, NL = 5;
   // set-A / set-T values of the CFS-PML auxiliaries loaded ahead in wide tiles
@@ -418,7 +424,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
                                 (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && je >= g.face[3].lo1));
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
-  const long bo = (long)member_slot(g) * g.bstride;
+  const long bo = (long)slot * g.bstride;
   const float* __restrict__ W = g.win + bo;
   float* __restrict__ wout = g.wout + bo;
   const float* __restrict__ mm = g.m + bo;
@@ -463,7 +469,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
   }
   AuxPre pre[RY];
 #pragma unroll
-  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + RS * q, k0, 3, comp[q]) : AuxPre{nullptr, -1, 0, 0};
+  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, slot, i, j0 + ty + RS * q, k0, 3, comp[q]) : AuxPre{nullptr, -1, 0, 0};
   unsigned kz = sz * (unsigned)k0;
   float hxTxx = ld(pTxx, hx, kz), hxTxy = ld(pTxy, hx, kz), hxTxz = ld(pTxz, hx, kz);
   float hyTxy = ld(pTxy, hy, kz), hyTyy = ld(pTyy, hy, kz), hyTyz = ld(pTyz, hy, kz);
@@ -564,7 +570,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, floa
         h[1] = rr[q] * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
         h[2] = rr[q] * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
         if constexpr (PML)
-          if (pml_here) pml_vel<KIND, FULLV>(g, i, j0 + ty + RS * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
+          if (pml_here) pml_vel<KIND, FULLV>(g, slot, i, j0 + ty + RS * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
                         rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
@@ -606,7 +612,7 @@ struct StrLds {
   static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, SIZE = 2 * 3 * PL;
 };
 template <int XF, int YF, int ZF, int KIND, int BYT, int RY, int TW, bool PML>
-__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, float* shbase) {
+__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* shbase) {
   constexpr int RS = BYT * (TX / TW), TY = RS * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH;
   float(*sh)[3 * PL] = reinterpret_cast<float(*)[3 * PL]>(shbase);  // Vx Vy Vz, double buffered
   const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * threadIdx.y, tid = threadIdx.x + TX * threadIdx.y;
@@ -617,7 +623,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
                                 (g.face[2].on && j0 <= g.face[2].hi1) || (g.face[3].on && je >= g.face[3].lo1));
   const long fs = g.fs;
   const unsigned sz = (unsigned)g.sz;
-  const long bo = (long)member_slot(g) * g.bstride;
+  const long bo = (long)slot * g.bstride;
   const float* __restrict__ W = g.win + bo;
   float* __restrict__ wout = g.wout + bo;
   const float* __restrict__ mm = g.m + bo;
@@ -654,7 +660,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
   }
   AuxPre pre[RY];
 #pragma unroll
-  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, i, j0 + ty + RS * q, k0, 0, comp[q]) : AuxPre{nullptr, -1, 0, 0};
+  for (int q = 0; q < RY; ++q) pre[q] = PML ? aux_pre(g, slot, i, j0 + ty + RS * q, k0, 0, comp[q]) : AuxPre{nullptr, -1, 0, 0};
   unsigned kz = sz * (unsigned)k0;
   float hx0 = ld(pV0, hx, kz), hx1 = ld(pV1, hx, kz), hx2 = ld(pV2, hx, kz);
   float hy0 = ld(pV0, hy, kz), hy1 = ld(pV1, hy, kz), hy2 = ld(pV2, hy, kz);
@@ -691,7 +697,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
         const long e = (TXX + f) * fs + c;
         float mv = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : mm + e) : 0.f;
         if (surf && comp[q] && KIND != 2 && (f == 2 || f == 4 || f == 5))
-          mv = g.msurf[(long)member_slot(g) * 3 * g.ps + (f == 2 ? 0 : (f == 4 ? 1 : 2)) * g.ps + cc[q]];
+          mv = g.msurf[(long)slot * 3 * g.ps + (f == 2 ? 0 : (f == 4 ? 1 : 2)) * g.ps + cc[q]];
         mo[q][f] = mv;
         to[q][f] = (comp[q] && KIND != 0) ? lds_nt(tt + e) : 0.f;
       }
@@ -761,7 +767,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, floa
         h[4] = 2.0f * miu * E13;
         h[5] = 2.0f * miu * E23;
         if constexpr (PML)
-          if (pml_here) pml_str<KIND>(g, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
+          if (pml_here) pml_str<KIND>(g, slot, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
                         sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
@@ -795,27 +801,29 @@ template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
 __global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) void k_vel(StageArgs g) {
   constexpr int SW = VelLds<BYT, RY, TX>::SIZE, SN = VelLds<BYT, RY, TXN>::SIZE;
   __shared__ float sh[SW > SN ? SW : SN];
-  const int L = item_index(g);
+  const ItemPos ip = item_pos(g);
+  const int L = ip.item, slot = g.slot0 + ip.member;
   const int4 it = g.items[L];
   if (L < g.p1)
-    vel_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, sh);
+    vel_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, slot, sh);
   else if (L < g.p2)
-    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, sh);
+    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, slot, sh);
   else
-    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, sh);
+    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, slot, sh);
 }
 template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
 __global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) void k_str(StageArgs g) {
   constexpr int SW = StrLds<BYT, RY, TX>::SIZE, SN = StrLds<BYT, RY, TXN>::SIZE;
   __shared__ float sh[SW > SN ? SW : SN];
-  const int L = item_index(g);
+  const ItemPos ip = item_pos(g);
+  const int L = ip.item, slot = g.slot0 + ip.member;
   const int4 it = g.items[L];
   if (L < g.p1)
-    str_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, sh);
+    str_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, slot, sh);
   else if (L < g.p2)
-    str_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, sh);
+    str_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, slot, sh);
   else
-    str_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, sh);
+    str_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, slot, sh);
 }
 
 }  // namespace fw
