@@ -11,9 +11,9 @@
 // TW x TY tile of (i, j) columns over a chunk of k planes and marches k.
 // Tiles are TW = 64 columns wide (a wave row is one 256-byte row segment),
 // except the narrow tiles at the x edge of the volume (the remainder of the
-// x extent after whole 64-column tiles), which are TW = 16 columns wide: a
-// wave then covers 4 rows of 16 columns, and no lanes idle on a 16-column
-// remainder.
+// x extent after whole 64-column tiles, when it is at most 32 columns),
+// which are TW = 16 columns wide: a wave then covers 4 rows of 16 columns,
+// and no lanes idle on a 16-column remainder.
 // The current k plane of every field that needs an x or y derivative is
 // staged in LDS with its one-sided halo (F: -1..+3, B: -3..+1), double
 // buffered so one barrier per plane suffices; z derivatives come from a
