@@ -93,10 +93,20 @@ __global__ void k_surface(SurfArgs g) {
   float* Txz = g.win + slot * g.bstride + TXZ * g.fs;
   float* Tyz = g.win + slot * g.bstride + TYZ * g.fs;
   float* msurf = g.msurf + slot * 3 * g.ps;
+  // every load before any store: the images above the surface and the
+  // stresses below it are distinct cells, and issuing the loads together
+  // takes one memory latency instead of one per image plane
+  float bzz[4], bxz[4], byz[4];
+#pragma unroll
+  for (int k = 0; k <= 3; ++k) {
+    bzz[k] = Tzz[cs - k * g.sz];
+    bxz[k] = Txz[cs - k * g.sz];
+    byz[k] = Tyz[cs - k * g.sz];
+  }
   if (g.save) {
-    msurf[c2] = Tzz[cs];
-    msurf[g.ps + c2] = Txz[cs];
-    msurf[2 * g.ps + c2] = Tyz[cs];
+    msurf[c2] = bzz[0];
+    msurf[g.ps + c2] = bxz[0];
+    msurf[2 * g.ps + c2] = byz[0];
   }
   float tx = 0.f, ty = 0.f, tz = 0.f;
   const int e = g.smap[slot * g.ps + c2];
@@ -110,9 +120,9 @@ __global__ void k_surface(SurfArgs g) {
   }
 #pragma unroll
   for (int k = 1; k <= 3; ++k) {
-    Tzz[cs + k * g.sz] = 2.0f * tz - Tzz[cs - k * g.sz];
-    Txz[cs + k * g.sz] = 2.0f * tx - Txz[cs - k * g.sz];
-    Tyz[cs + k * g.sz] = 2.0f * ty - Tyz[cs - k * g.sz];
+    Tzz[cs + k * g.sz] = 2.0f * tz - bzz[k];
+    Txz[cs + k * g.sz] = 2.0f * tx - bxz[k];
+    Tyz[cs + k * g.sz] = 2.0f * ty - byz[k];
   }
   Tzz[cs] = tz;
   Txz[cs] = tx;
@@ -207,13 +217,21 @@ __global__ void k_src(SrcArgs g) {
   }
   float* wout = g.wout + slot * g.bstride;
   float* t = g.t + slot * g.bstride;
+  // all loads, then all stores (wout and t are distinct register sets): one
+  // memory latency instead of one per field
+  float wv[9], tv[9];
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    wv[f] = wout[f * g.fs + c];
+    tv[f] = g.kind == 2 ? 0.f : t[f * g.fs + c];
+  }
 #pragma unroll
   for (int f = 0; f < 9; ++f) {
     if (g.kind == 2) {
-      wout[f * g.fs + c] += g.b * dh[f];
+      wout[f * g.fs + c] = wv[f] + g.b * dh[f];
     } else {
-      wout[f * g.fs + c] += g.a * dh[f];
-      t[f * g.fs + c] += g.b * dh[f];
+      wout[f * g.fs + c] = wv[f] + g.a * dh[f];
+      t[f * g.fs + c] = tv[f] + g.b * dh[f];
     }
   }
 }
