@@ -446,6 +446,7 @@ struct fw_solver_s {
   int tilecfg = 0;            // stage-kernel tile shape index
   int4* d_items = nullptr;    // stage work items: PML-shell items, then interior items
   int n_narrow = 0, n_lean = 0, n_pml = 0;
+  int xg = 1;  // stage-kernel items per XCD run
   // host copies
   std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
   std::vector<float> hlam, hmu, hrho;
@@ -1228,6 +1229,7 @@ static int build_items(fw_handle h) {
   hipFree(h->d_items);
   h->d_items = nullptr;
   h->n_narrow = (int)narrow.size();
+  h->xg = std::max(1, (int)xt.size());
   h->n_lean = (int)lean.size();
   h->n_pml = (int)pml.size();
   std::vector<int4> all(narrow);
@@ -1300,6 +1302,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   g.nitems = h->n_narrow + h->n_pml + h->n_lean;
   g.p1 = h->n_narrow;
   g.p2 = h->n_narrow + h->n_pml;
+  g.xg = h->xg;
   g.bstride = h->bstride;
   const int W = h->width > 0 ? h->width : nact;
   for (int s0 = 0; s0 < nact && g.nitems > 0; s0 += W) {  // members s0 .. s0+W-1 in one launch
