@@ -1232,7 +1232,7 @@ static int build_items(fw_handle h) {
   hipFree(h->d_items);
   h->d_items = nullptr;
   h->n_narrow = (int)narrow.size();
-  h->xg = std::max(1, (int)xt.size());
+  h->xg = std::max(1, ((int)xt.size() + 1) / 2);  // half a tile row (3 of 6 measured 1% faster than 1, 2, 4, 6, 12)
   h->n_lean = (int)lean.size();
   h->n_pml = (int)pml.size();
   std::vector<int4> all(narrow);

@@ -96,16 +96,16 @@ struct StageArgs {
   // (per face)
   long bstride;
   int slot0, nm;
-  int xg;  // items per XCD run (the wide tiles of a row)
+  int xg;  // items per XCD run (half the wide tiles of a row)
 };
 
 // Blocks are dealt round-robin to the 8 XCDs.  The (work item, member)
 // pairs of a launch are ordered item-major (the members of one item run side
 // by side on one XCD and share its media reads in L2), and the items by
 // chunk, then tile row, then tile column.  Runs of xg consecutive pairs (xg:
-// the wide tiles of a row times the members) go to one XCD, the next run to
-// the next XCD: neighbouring tiles of a row share their x halos in one L2,
-// and every XCD gets an even share of every kind of item.  (Giving each XCD
+// half the wide tiles of a row, times the members) go to one XCD, the next
+// run to the next XCD: neighbouring tiles of a row share their x halos in one
+// L2, and every XCD gets an even share of every kind of item.  (Giving each XCD
 // one contiguous eighth of the list instead keeps more halos in one L2 but
 // measures 4-5% slower: the eighths differ in cost.)  The item list holds
 // three parts, dispatched in this order: the narrow edge tiles [0, p1), the
