@@ -1211,9 +1211,8 @@ static int build_items(fw_handle h) {
       if (x.pml || y.pml) {
         add_range(pml, x, y, h->nk1, h->nk2);
       } else {
-        // chunks over the whole depth; one that holds planes of a z slab runs
-        // the PML body (a short slab-only chunk would pay the window start
-        // of a whole chunk)
+        // the same chunks over the whole depth as every other tile; one that
+        // holds planes of a z slab runs the PML body
         for (int k0 = h->nk1; k0 <= h->nk2; k0 += cs) {
           const int k1 = std::min(k0 + cs - 1, h->nk2);
           const int4 it = make_int4(x.lo | (x.n << 16), y.lo | (y.n << 16), k0, k1);
