@@ -43,12 +43,12 @@ def _p(a):
     return C.c_void_p(a if isinstance(a, int) else a.ptr)
 
 
-def init_weights(seed=0):
+def init_weights(seed=0, layers=LAYERS):
     """He-scaled normal weights, zero biases: list of (w, b) host arrays.
     Conv weight [Cout][Cin][K^3]; transposed conv weight [Cin][Cout][K^3]."""
     rng = np.random.default_rng(seed)
     out = []
-    for kind, ci, co, di, do, k, s, p, a in LAYERS:
+    for kind, ci, co, di, do, k, s, p, a in layers:
         fan_in = ci * k ** 3 if kind == "conv" else ci * k ** 3 // (s ** 3)
         std = np.sqrt(2.0 / fan_in)
         shape = (co, ci, k ** 3) if kind == "conv" else (ci, co, k ** 3)
@@ -57,15 +57,19 @@ def init_weights(seed=0):
 
 
 class AE:
-    """Weights and Adam state in HBM; forward / backward / step on batches."""
+    """Weights and Adam state in HBM; forward / backward / step on batches.
+    layers: the network (default LAYERS), n_enc: its encoder layers."""
 
-    def __init__(self, weights=None, seed=0, nchunk=64):
+    def __init__(self, weights=None, seed=0, nchunk=64, layers=LAYERS, n_enc=N_ENC):
         self.L = _lib.lib()
-        self.host = weights if weights is not None else init_weights(seed)
+        self.layers, self.n_enc = tuple(layers), n_enc
+        _, _, co, _, do, _, _, _, _ = self.layers[n_enc - 1]
+        self.latent = co * do ** 3
+        self.host = weights if weights is not None else init_weights(seed, self.layers)
         self.w = [DeviceArray.from_host(w) for w, _ in self.host]
         self.b = [DeviceArray.from_host(b) for _, b in self.host]
         # gradient buffer per layer: dw then db (the layout fwae_conv_bwd_weight writes)
-        self.g = [DeviceArray((w.numel() + max(L[1], L[2]),), np.float32) for w, L in zip(self.w, LAYERS)]
+        self.g = [DeviceArray((w.numel() + max(L[1], L[2]),), np.float32) for w, L in zip(self.w, self.layers)]
         self.m = [DeviceArray.from_host(np.zeros(g.numel(), np.float32)) for g in self.g]
         self.v = [DeviceArray.from_host(np.zeros(g.numel(), np.float32)) for g in self.g]
         self.nchunk = nchunk
@@ -83,7 +87,7 @@ class AE:
 
     # -- forward ----------------------------------------------------------------
     def _layer_fwd(self, q, x, n, y):
-        kind, ci, co, di, do, k, s, p, a = LAYERS[q]
+        kind, ci, co, di, do, k, s, p, a = self.layers[q]
         if kind == "conv":
             kcheck(self.L.fwae_conv_fwd(n, ci, co, di, do, k, s, p, _p(x), _p(self.w[q]), _p(self.b[q]), _p(y), None),
                    "fwae_conv_fwd")
@@ -92,13 +96,13 @@ class AE:
                                              None), "fwae_conv_bwd_data")
         kcheck(self.L.fwae_act_fwd(y.numel(), a, _p(y), None), "fwae_act_fwd")
 
-    def forward(self, x, n, layers=range(len(LAYERS)), tag="t"):
-        """x: DeviceArray of n inputs of the first layer in `layers`; returns
-        the last layer's output (activations kept for backward)."""
+    def forward(self, x, n, layers=None, tag="t"):
+        """x: DeviceArray of n inputs of the first layer in `layers` (default:
+        all); returns the last layer's output (activations kept for backward)."""
         cur = x
         self._inputs = {}
-        for q in layers:
-            kind, ci, co, di, do, k, s, p, a = LAYERS[q]
+        for q in (range(len(self.layers)) if layers is None else layers):
+            kind, ci, co, di, do, k, s, p, a = self.layers[q]
             y = self._buf((tag, q), (n, co, do, do, do))
             self._inputs[q] = cur
             self._layer_fwd(q, cur, n, y)
@@ -106,17 +110,17 @@ class AE:
         return cur
 
     def encode(self, x, n):
-        return self.forward(x, n, range(N_ENC), tag="e")
+        return self.forward(x, n, range(self.n_enc), tag="e")
 
     def decode(self, z, n):
-        return self.forward(z, n, range(N_ENC, len(LAYERS)), tag="d")
+        return self.forward(z, n, range(self.n_enc, len(self.layers)), tag="d")
 
     # -- backward ---------------------------------------------------------------
     def backward(self, dy, n):
         """dy: gradient of the loss w.r.t. the network output (overwritten)."""
         cur = dy
-        for q in reversed(range(len(LAYERS))):
-            kind, ci, co, di, do, k, s, p, a = LAYERS[q]
+        for q in reversed(range(len(self.layers))):
+            kind, ci, co, di, do, k, s, p, a = self.layers[q]
             y = self._acts[("t", q)]
             x = self._inputs[q]
             kcheck(self.L.fwae_act_bwd(y.numel(), a, _p(y), _p(cur), None), "fwae_act_bwd")
@@ -145,7 +149,7 @@ class AE:
         """Bias gradient of a transposed layer: sum of dz per output channel,
         computed as the weight gradient of a 1x1x1 convolution from an
         all-ones input channel."""
-        kind, ci, co, di, do, k, s, p, a = LAYERS[q]
+        kind, ci, co, di, do, k, s, p, a = self.layers[q]
         ones = self._buf(("ones", q), (n, 1, do, do, do))
         if not getattr(ones, "_filled", False):
             kKK = np.ones((n, 1, do, do, do), np.float32)
@@ -160,7 +164,7 @@ class AE:
 
     def step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         self.step_no += 1
-        for q in range(len(LAYERS)):
+        for q in range(len(self.layers)):
             nw = self.w[q].numel()
             kcheck(self.L.fwae_adam(nw, _p(self.w[q]), _p(self.g[q]), _p(self.m[q]), _p(self.v[q]), lr, b1, b2, eps,
                                     self.step_no, None), "fwae_adam")
@@ -226,7 +230,7 @@ def reconstruct(ae, blocks, n, batch=64):
     """Decoded scaled blocks (host) and the latents (host, BF16-rounded by the network)."""
     host = blocks.to_host().reshape(n, -1)
     out = np.zeros_like(host)
-    lat = np.zeros((n, LATENT), np.float32)
+    lat = np.zeros((n, ae.latent), np.float32)
     x = DeviceArray((batch, NCOMP, 16, 16, 16), np.float32)
     for a in range(0, n, batch):
         m = min(batch, n - a)
@@ -268,7 +272,7 @@ def compress(ae, records, subc, threshold=1e-3):
     err = block_errors(true, dec, scales)
     zero = scales.max(axis=1) == 0
     acc = (err <= threshold) | zero
-    stored = int(acc.sum()) * (LATENT * 2 + NCOMP * 4) + int((~acc).sum()) * NCOMP * BLOCK * 4
+    stored = int(acc.sum()) * (ae.latent * 2 + NCOMP * 4) + int((~acc).sum()) * NCOMP * BLOCK * 4
     return Compressed(acc, err, stored, n * NCOMP * BLOCK * 4), dec, true, scales
 
 
