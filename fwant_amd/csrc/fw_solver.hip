@@ -1608,8 +1608,15 @@ int fw_get_wavefield(fw_handle h, int f, float* out) {
 int fw_set_wavefield(fw_handle h, int f, const float* in) {
   if (!h || f < 0 || f >= 9 || !in) return fail(h, FW_ERR_ARG, "fw_set_wavefield: bad argument");
   if (!h->ready) return fail(h, FW_ERR_STATE, "fw_set_wavefield: call fw_init first");
-  int rc = upload_volume(h, h->set[0] + (size_t)sel_slot(h) * h->bstride + (size_t)f * h->fs, in);
+  const size_t off = (size_t)sel_slot(h) * h->bstride + (size_t)f * h->fs;
+  int rc = upload_volume(h, h->set[0] + off, in);
   if (rc) return rc;
+  // The stages read their input from sets A, B and C in turn, and no stage
+  // writes the x / y ghost layers: the array goes to all three, so that its
+  // ghost values are seen by every stage, as with the reference's single
+  // array per field (the interiors of B and C are written before any read).
+  for (int q = 1; q <= 2; ++q)
+    HIPCHK(hipMemcpyAsync(h->set[q] + off, h->set[0] + off, sizeof(float) * h->fs, hipMemcpyDeviceToDevice, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   return FW_OK;
 }
