@@ -150,7 +150,9 @@ int fw_get_snapshot_packed(fw_handle h, int snap_id, uint16_t *codes, float *sca
 int fw_snapshot_packed_device(fw_handle h, int snap_id, uint16_t **codes, float **scales, int *nrecords,
                               long *npoints, long *nblocks);
 /* Wavefield state (between steps): ghosted nx*ny*nz host arrays.  Replaces
- * io_rest_export / io_rest_import's array transfer (mod_io.F90:342-427). */
+ * io_rest_export / io_rest_import's array transfer (mod_io.F90:342-427).
+ * Ghost values set here are seen by every later stage (no stage writes the
+ * x / y ghosts), as with the reference's single array per field. */
 int fw_get_wavefield(fw_handle h, int field, float *out);
 int fw_set_wavefield(fw_handle h, int field, const float *in);
 
