@@ -19,6 +19,8 @@ Prints ONE JSON line:
              profiles/stage_traffic_gradient_400x400x200.json)
   cpu_baseline  the FP32 C oracle (OpenMP) on the host cores, timed on a
              bounded sample of the same workload (2 full steps)
+  accuracy   receiver records of the two-layer 100x100x50 workload (1008
+             steps) against the committed FP32 / FP64 CPU records
   pipeline   station-pair kernels/hour, plain (one FP32 solve per station)
              vs batched + BF16 SGT (one batched solve with step graphs and
              the autotuner), on a 256x256x128 sample: 4 virtual sources of
@@ -145,6 +147,34 @@ def pipeline_sample():
             "kernel_rel_l2_bf16_vs_fp32": max(errs), "kernels_compared": len(errs)}
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden", "two_layer_100x100x50.npz")
+
+
+def accuracy_sample():
+    """The GPU's receiver records on the two-layer 100x100x50 workload (the
+    full 1008 steps) against the committed FP32 and FP64 CPU records of the
+    same inputs (tests/golden); the largest relative L2 over the six
+    components that do not vanish at the traction-free surface."""
+    import numpy as np
+    from fwant_amd import models
+    from fwant_amd.solver import Solver
+    d = np.load(GOLDEN)
+    m = models.two_layer_100x100x50()
+    s = Solver(m)
+    s.run(int(d["steps"]))
+    g = s.receivers().astype(np.float64)
+    s.close()
+    r32, r64 = d["rec32"].astype(np.float64), d["rec64"].astype(np.float64)
+    e32 = e64 = o64 = 0.0
+    for f in (0, 1, 2, 3, 4, 6):  # Vx Vy Vz Txx Tyy Txy
+        e32 = max(e32, np.linalg.norm(g[..., f] - r32[..., f]) / np.linalg.norm(r32[..., f]))
+        e64 = max(e64, np.linalg.norm(g[..., f] - r64[..., f]) / np.linalg.norm(r64[..., f]))
+        o64 = max(o64, np.linalg.norm(r32[..., f] - r64[..., f]) / np.linalg.norm(r64[..., f]))
+    log(f"accuracy: vs FP32 CPU {e32:.2e}, vs FP64 CPU {e64:.2e} (FP32 CPU vs FP64 {o64:.2e})")
+    return {"workload": "two_layer_100x100x50, 1008 steps, 16 surface receivers",
+            "rel_l2_vs_fp32_cpu": e32, "rel_l2_vs_fp64_cpu": e64, "fp32_cpu_vs_fp64_cpu": o64}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +186,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--traffic", type=str, default=TRAFFIC_JSON,
                     help="JSON file with measured HBM bytes per RK stage (written by --summarize-pmc)")
     ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "OUT"),
@@ -240,6 +271,7 @@ def main():
         o.close()
 
     pipe = None if args.no_pipeline else pipeline_sample()
+    acc = None if args.no_accuracy else accuracy_sample()
 
     out = {
         "metric": "cell-updates/s + % HBM peak; batched+compressed vs plain at ≤1% kernel err",
@@ -264,6 +296,7 @@ def main():
                      "mean_launch_us": avg_ms * 1e3},
         "cpu_baseline": cpu,
         "pipeline": pipe,
+        "accuracy": acc,
     }
     print(json.dumps(out), flush=True)
 
