@@ -50,7 +50,9 @@ def test_stress2strain(dev):
         assert rel(got[c], ref[c]) < 1e-6
 
 
-@pytest.mark.parametrize("knt", [5, 64, 203])
+# knt >= 320 runs the LDS-tiled kernel (64-point by 32-output tiles; 1000
+# points leave a partial point tile, 333 and 411 partial output tiles)
+@pytest.mark.parametrize("knt", [5, 64, 203, 333, 411])
 def test_scatter(dev, knt):
     from fwant_amd import kernel
     from oracle.pyoracle import Lib

@@ -18,12 +18,13 @@ def P(lib, a):
     return a.ctypes.data_as(C.POINTER(lib.R))
 
 
-def test_lamu_and_density_scatter(gpu_lib):
+@pytest.mark.parametrize("knt", [61, 337])  # 337: the LDS-tiled scatter kernel
+def test_lamu_and_density_scatter(gpu_lib, knt):
     from fwant_amd import kernel
     from fwant_amd._lib import DeviceArray
     from oracle.pyoracle import Lib
     rng = np.random.default_rng(11)
-    npt, knt, dt = 700, 61, 0.3
+    npt, dt = 700, 0.3
     R = rng.standard_normal((6, knt, npt)).astype(np.float32)
     S = rng.standard_normal((6, knt, npt)).astype(np.float32)
     lam = rng.uniform(1, 3, npt).astype(np.float32)
