@@ -51,9 +51,14 @@ thread_local std::string g_create_err;
 
 // stage-kernel tile shapes (threads 64 x BYT, RY rows per thread): selected
 // per handle (FWANT_TILE or the autotuner), all 16-row tiles
-constexpr int NTILECFG = 2;
-constexpr int TILE_BYT[NTILECFG] = {4, 8};  // waves per block
-constexpr int TILE_RY[NTILECFG] = {2, 2};   // rows per thread: 8-row and 16-row tiles
+// Stage-kernel tile shapes (the autotuner picks among them; the first is the
+// default): 4-row tiles of 4 waves with 1 row per thread, 8-row tiles of 4
+// waves with 2 rows per thread, 16-row tiles of 8 waves, 8-row tiles of 8
+// waves with 1 row per thread.  At 400x400x200 with PML: 1250, 1293, 1481,
+// 1259 us per stage.
+constexpr int NTILECFG = 4;
+constexpr int TILE_BYT[NTILECFG] = {4, 4, 8, 8};  // waves per block
+constexpr int TILE_RY[NTILECFG] = {1, 2, 2, 1};   // rows per thread
 
 // ---------------------------------------------------------------------------
 // Free-surface stress imaging (free_timg, mod_macdrp.F90:3414-3436) on the
@@ -1149,11 +1154,11 @@ static StageFn str_fn() {
 #define VTAB(C) {VROW(0, C), VROW(1, C), VROW(2, C), VROW(3, C), VROW(4, C), VROW(5, C), VROW(6, C), VROW(7, C)}
 #define STAB(C) {SROW(0, C), SROW(1, C), SROW(2, C), SROW(3, C), SROW(4, C), SROW(5, C), SROW(6, C), SROW(7, C)}
 static StageFn vel_table(int c, int p, int kind) {
-  static const StageFn T[NTILECFG][8][3] = {VTAB(0), VTAB(1)};
+  static const StageFn T[NTILECFG][8][3] = {VTAB(0), VTAB(1), VTAB(2), VTAB(3)};
   return T[c][p][kind];
 }
 static StageFn str_table(int c, int p, int kind) {
-  static const StageFn T[NTILECFG][8][3] = {STAB(0), STAB(1)};
+  static const StageFn T[NTILECFG][8][3] = {STAB(0), STAB(1), STAB(2), STAB(3)};
   return T[c][p][kind];
 }
 
@@ -1194,10 +1199,12 @@ static int build_items(fw_handle h) {
   const int zhi = F[5].on ? F[5].lo[2] : h->nk2 + 1;  // first plane of the z2 slab
   const int nk = h->nk2 - h->nk1 + 1;
   const int ntile = (int)(xt.size() * yt.size() + xn.size() * ytn.size());
-  // about 12288 items of at least 4 planes: short items keep every XCD busy
-  // to the end of the launch, which outweighs the z-window restarts they
-  // add (measured over 700-16384 items and 4-16 planes per item)
-  int nch = std::max(1, std::min(std::max(1, nk / 4), (12288 + ntile - 1) / ntile));
+  // items of about 5 planes (at least 4), 98304 / TY of them at 400x400x200
+  // (24576 for 4-row tiles): short items keep every XCD busy to the end of
+  // the launch, which outweighs the z-window restarts they add (measured
+  // over 700-36864 items and 4-16 planes per item)
+  const int target = 98304 / TY;
+  int nch = std::max(1, std::min(std::max(1, nk / 4), (target + ntile - 1) / ntile));
   const int cs = std::max(1, (nk + nch - 1) / nch);
   std::vector<int4> narrow, lean, pml;
   auto add_range = [&](std::vector<int4>& v, const Tile& x, const Tile& y, int ka, int kb) {

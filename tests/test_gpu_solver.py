@@ -49,7 +49,7 @@ def test_solver_parity(gpu_lib, case, monkeypatch):
     elif case == "narrow_x1":
         m = models.small_test(ni=140, nj=30, nk=20, nt=50, seed=17, freenode=False)
     elif case == "narrow_tall":
-        monkeypatch.setenv("FWANT_TILE", "1")
+        monkeypatch.setenv("FWANT_TILE", "2")
         m = models.small_test(ni=76, nj=70, nk=20, nt=60, seed=16)
     elif case == "free_surface":
         m = models.small_test(nt=96)
@@ -136,7 +136,7 @@ def test_restart_from_state(gpu_lib):
     s2.close()
 
 
-@pytest.mark.parametrize("case", ["narrow_pml6", "water_narrow", "tall_tiles"])
+@pytest.mark.parametrize("case", ["narrow_pml6", "water_narrow", "tall_tiles", "eight_row", "eight_row_8w"])
 def test_random_state_parity(gpu_lib, case, monkeypatch):
     """From the same random state (ghost layers included) the GPU and the
     oracle agree after two F/B cycles: every cell, slab, edge tile and fluid
@@ -147,9 +147,12 @@ def test_random_state_parity(gpu_lib, case, monkeypatch):
         m = models.small_test(ni=96, nj=40, nk=24, nt=16, seed=31, freenode=False)
     elif case == "water_narrow":
         m = models.small_test(ni=76, nj=36, nk=24, nt=16, seed=32, water=True)
-    else:
-        monkeypatch.setenv("FWANT_TILE", "1")
+    elif case == "tall_tiles":  # 16-row tiles
+        monkeypatch.setenv("FWANT_TILE", "2")
         m = models.small_test(ni=80, nj=70, nk=20, nt=16, seed=33)
+    else:  # 8-row tiles: 4 waves of 2 rows, 8 waves of 1 row
+        monkeypatch.setenv("FWANT_TILE", "1" if case == "eight_row" else "3")
+        m = models.small_test(ni=76, nj=50, nk=22, nt=16, seed=34, freenode=case == "eight_row")
     print(f"case={case} grid={m.ni}x{m.nj}x{m.nk} PML {m.abs_number}", flush=True)
     s = Solver(m, max_records=32)
     o = OracleSolver(m, 32, metric=s.metric)
