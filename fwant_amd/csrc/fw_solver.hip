@@ -1729,22 +1729,28 @@ int fw_autotune(fw_handle h, int trial_steps, int* tile, int* width, float* ms_p
   HIPCHK(hipEventCreate(&e1));
   float best = 3.4e38f;
   int bt = h->tilecfg, bw = h->width, rc = FW_OK;
-  for (int tc = 0; tc < NTILECFG && !rc; ++tc) {
+  auto trial = [&](int tc, int w) -> int {
     h->tilecfg = tc;
-    if ((rc = build_items(h))) break;
-    for (int w : widths) {
-      if ((rc = fw_set_batch_width(h, w)) || (rc = zero_state(h))) break;
-      reset_records(h);
-      if ((rc = fw_run(h, 0, 1))) break;  // warm-up (graph capture, first touch)
-      hipEventRecord(e0, h->stream);
-      if ((rc = fw_run(h, 1, trial_steps))) break;
-      hipEventRecord(e1, h->stream);
-      hipEventSynchronize(e1);
-      float ms = 0.f;
-      hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) { best = ms; bt = tc; bw = w; }
-    }
-  }
+    int r = build_items(h);
+    if (!r) r = fw_set_batch_width(h, w);
+    if (!r) r = zero_state(h);
+    if (r) return r;
+    reset_records(h);
+    if ((r = fw_run(h, 0, 1))) return r;  // warm-up (graph capture, first touch)
+    hipEventRecord(e0, h->stream);
+    if ((r = fw_run(h, 1, trial_steps))) return r;
+    hipEventRecord(e1, h->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) { best = ms; bt = tc; bw = w; }
+    return FW_OK;
+  };
+  // the tile shapes with all running members in one launch, then the batch
+  // widths with the fastest shape (NTILECFG + widths - 1 trials)
+  for (int tc = 0; tc < NTILECFG && !rc; ++tc) rc = trial(tc, 0);
+  const int tbest = bt;
+  for (size_t q = 1; q < widths.size() && !rc; ++q) rc = trial(tbest, widths[q]);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   h->timing = timing;

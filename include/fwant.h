@@ -174,9 +174,10 @@ int fw_set_batch_width(fw_handle h, int width);
  * slots come from a device-side step clock, so replays need no host
  * arguments.  Results are identical to the launch-by-launch path. */
 int fw_use_graphs(fw_handle h, int on);
-/* Time trial_steps steps for every stage-kernel tile shape and batch width
- * (powers of two below nbatch, and all), keep the fastest, and restore the
- * zero initial state with empty records.  Call after fw_init. */
+/* Time trial_steps steps for every stage-kernel tile shape (all running
+ * members in one launch), then for the batch widths (powers of two below
+ * nbatch) with the fastest shape; keep the fastest, and restore the zero
+ * initial state with empty records.  Call after fw_init. */
 int fw_autotune(fw_handle h, int trial_steps, int *tile, int *width, float *ms_per_step);
 
 /* Restart.  The reference restarts from the 9 wavefields alone: its PML
