@@ -801,9 +801,11 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
 // body (with the CFS-PML corrections), the wide PML-shell tiles the PML
 // body, the wide interior tiles the lean body.  At most 168 VGPRs (3 waves
 // per SIMD): the inner RK kind of k_vel then keeps a few values in scratch,
-// which measures faster than 2 waves per SIMD.
+// which measures faster than 2 waves per SIMD.  With 4-row tiles the first
+// and last RK kinds of k_vel fit 96 VGPRs (5 waves per SIMD; 0.4% faster
+// per stage, the inner kind and k_str spill too much there).
 template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
-__global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) void k_vel(StageArgs g) {
+__global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(BYT * RY == 4 && KIND != 1 ? 5 : 3))) void k_vel(StageArgs g) {
   constexpr int SW = VelLds<BYT, RY, TX>::SIZE, SN = VelLds<BYT, RY, TXN>::SIZE;
   __shared__ float sh[SW > SN ? SW : SN];
   const ItemPos ip = item_pos(g);
