@@ -195,6 +195,8 @@ def main():
                     help="print a kernel-duration table from a rocprofv3 --kernel-trace database and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
+    ap.add_argument("--pipeline-only", action="store_true",
+                    help="run only the kernels/hour pipeline sample and print its JSON (for profiling it alone)")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
@@ -211,6 +213,9 @@ def main():
             b.free()
         device_sync()
         log("copy calibration: 4 device copies of 1 GiB (read 1 GiB + write 1 GiB each)")
+        return
+    if args.pipeline_only:
+        print(json.dumps(pipeline_sample()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
