@@ -1,16 +1,23 @@
 """Build libfwant.so (HIP, gfx950) in-tree.
 
-The library is compiled from fwant_amd/csrc/*.hip with hipcc for gfx950 only.
+The library is compiled from fwant_amd/csrc/*.hip with hipcc for gfx950 only,
+one object per translation unit, in parallel: the stage kernels
+(fw_stage_inst.hip) are compiled once per (kernel, tile shape).  Objects go
+to fwant_amd/_build/, the library to fwant_amd/libfwant.so.
 Packed-FP32 instruction selection is disabled (-packed-fp32-ops).
 """
+import concurrent.futures as cf
 import os
 import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfwant.so")
 SOURCES = ["fw_solver.hip", "fw_kernel.hip", "fw_inv.hip", "fw_ae.hip"]
+STAGE_SRC = "fw_stage_inst.hip"
+NTILE = 4
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 FLAGS = [
@@ -18,7 +25,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
     "-Wno-unused-value",
@@ -52,20 +58,61 @@ def build_fortran(verbose: bool = True) -> str:
     return DRIVER
 
 
-def build(verbose: bool = True) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+def _units():
+    """(object path, source path, extra flags) of every translation unit."""
+    u = [(os.path.join(BUILD, s.replace(".hip", ".o")), os.path.join(CSRC, s), []) for s in SOURCES]
+    for k, name in ((0, "vel"), (1, "str")):
+        for t in range(NTILE):
+            u.append((os.path.join(BUILD, f"fw_stage_{name}{t}.o"), os.path.join(CSRC, STAGE_SRC),
+                      [f"-DFW_KERNEL={k}", f"-DFW_TILE={t}"]))
+    return u
+
+
+def _stale(obj, src):
+    """True when obj is older than its source or any header of csrc/ or include/."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    deps.append(os.path.join(HERE, "..", "include", "fwant.h"))
+    return any(os.path.exists(d) and os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = True, force: bool = True, jobs: int = None) -> str:
+    """Compile every translation unit (all of them when force, else the stale
+    ones) in parallel and link libfwant.so."""
+    os.makedirs(BUILD, exist_ok=True)
+    units = [u for u in _units() if force or _stale(u[0], u[1])]
+    jobs = jobs or max(1, min(len(units) or 1, os.cpu_count() or 1))
+
+    def compile_one(u):
+        obj, src, extra = u
+        cmd = [HIPCC, *FLAGS, *extra, "-c", "-o", obj + ".tmp", src]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            return cmd, r.stdout + r.stderr
+        os.replace(obj + ".tmp", obj)
+        return cmd, None
+
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for cmd, err in ex.map(compile_one, units):
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            if err is not None:
+                sys.stderr.write(err)
+                raise RuntimeError("hipcc failed building " + cmd[-1])
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *[u[0] for u in _units()]]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libfwant.so")
+        raise RuntimeError("hipcc failed linking libfwant.so")
     os.replace(tmp, LIB)
     return LIB
 
 
 if __name__ == "__main__":
-    build()
+    build(force="--changed" not in sys.argv)
     build_fortran()

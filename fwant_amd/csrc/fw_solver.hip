@@ -49,17 +49,6 @@ thread_local std::string g_create_err;
     }                                                                             \
   } while (0)
 
-// stage-kernel tile shapes (threads 64 x BYT, RY rows per thread): selected
-// per handle (FWANT_TILE or the autotuner), all 16-row tiles
-// Stage-kernel tile shapes (the autotuner picks among them; the first is the
-// default): 4-row tiles of 4 waves with 1 row per thread, 8-row tiles of 4
-// waves with 2 rows per thread, 16-row tiles of 8 waves, 8-row tiles of 8
-// waves with 1 row per thread.  At 400x400x200 with PML: 1250, 1293, 1481,
-// 1259 us per stage.
-constexpr int NTILECFG = 4;
-constexpr int TILE_BYT[NTILECFG] = {4, 4, 8, 8};  // waves per block
-constexpr int TILE_RY[NTILECFG] = {1, 2, 2, 1};   // rows per thread
-
 // ---------------------------------------------------------------------------
 // Free-surface stress imaging (free_timg, mod_macdrp.F90:3414-3436) on the
 // stage input, with the surface traction of surface force sources
@@ -1137,29 +1126,16 @@ int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Kernel dispatch tables over the 8 F/B patterns and 3 RK kinds.
+// Kernel dispatch over the tile shapes, 8 F/B patterns and 3 RK kinds (the
+// instantiations live in fw_stage_inst.hip).
 // ---------------------------------------------------------------------------
-typedef void (*StageFn)(StageArgs);
-
-template <int P, int K, int C>
-static StageFn vel_fn() {
-  return k_vel<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_BYT[C], TILE_RY[C]>;
-}
-template <int P, int K, int C>
-static StageFn str_fn() {
-  return k_str<(P >> 2) & 1, (P >> 1) & 1, P & 1, K, TILE_BYT[C], TILE_RY[C]>;
-}
-#define VROW(P, C) {vel_fn<P, 0, C>(), vel_fn<P, 1, C>(), vel_fn<P, 2, C>()}
-#define SROW(P, C) {str_fn<P, 0, C>(), str_fn<P, 1, C>(), str_fn<P, 2, C>()}
-#define VTAB(C) {VROW(0, C), VROW(1, C), VROW(2, C), VROW(3, C), VROW(4, C), VROW(5, C), VROW(6, C), VROW(7, C)}
-#define STAB(C) {SROW(0, C), SROW(1, C), SROW(2, C), SROW(3, C), SROW(4, C), SROW(5, C), SROW(6, C), SROW(7, C)}
 static StageFn vel_table(int c, int p, int kind) {
-  static const StageFn T[NTILECFG][8][3] = {VTAB(0), VTAB(1), VTAB(2), VTAB(3)};
-  return T[c][p][kind];
+  static StageFn (*const T[NTILECFG])(int, int) = {stage_fn_v0, stage_fn_v1, stage_fn_v2, stage_fn_v3};
+  return T[c](p, kind);
 }
 static StageFn str_table(int c, int p, int kind) {
-  static const StageFn T[NTILECFG][8][3] = {STAB(0), STAB(1), STAB(2), STAB(3)};
-  return T[c][p][kind];
+  static StageFn (*const T[NTILECFG])(int, int) = {stage_fn_s0, stage_fn_s1, stage_fn_s2, stage_fn_s3};
+  return T[c](p, kind);
 }
 
 // Work items of the stage kernels: tiles over chunks of k planes in three

@@ -833,4 +833,26 @@ __global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) vo
     str_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, slot, sh);
 }
 
+// Stage-kernel tile shapes (the autotuner picks among them; the first is the
+// default): 4-row tiles of 4 waves with 1 row per thread, 8-row tiles of 4
+// waves with 2 rows per thread, 16-row tiles of 8 waves, 8-row tiles of 8
+// waves with 1 row per thread.  At 400x400x200 with PML: 1250, 1293, 1481,
+// 1259 us per stage.
+constexpr int NTILECFG = 4;
+constexpr int TILE_BYT[NTILECFG] = {4, 4, 8, 8};  // waves per block
+constexpr int TILE_RY[NTILECFG] = {1, 2, 2, 1};   // rows per thread
+
+// The 8 F/B patterns x 3 RK kinds of one stage kernel (0: k_vel, 1: k_str)
+// and tile shape are instantiated in a translation unit of their own
+// (fw_stage_inst.hip, compiled once per kernel and shape, in parallel).
+typedef void (*StageFn)(StageArgs);
+StageFn stage_fn_v0(int pat, int kind);
+StageFn stage_fn_v1(int pat, int kind);
+StageFn stage_fn_v2(int pat, int kind);
+StageFn stage_fn_v3(int pat, int kind);
+StageFn stage_fn_s0(int pat, int kind);
+StageFn stage_fn_s1(int pat, int kind);
+StageFn stage_fn_s2(int pat, int kind);
+StageFn stage_fn_s3(int pat, int kind);
+
 }  // namespace fw

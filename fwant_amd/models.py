@@ -225,10 +225,11 @@ def gradient_400x400x200(nt=3000, ni=400, nj=400, nk=200, h=500.0, seed=3):
 
 
 def small_test(ni=40, nj=36, nk=32, nt=120, h=1000.0, seed=7, freenode=True, pml=8,
-               moment=True, colat0=np.pi / 2, r_top=R_EARTH, dt=0.05, water=False):
+               moment=True, colat0=np.pi / 2, r_top=R_EARTH, dt=0.05, water=False, fscale=1.0):
     """Reduced heterogeneous case exercising every path: random smooth media,
     PML on 5 (or 6) sides, a smoothed surface force with two STF windows,
-    an interior moment source, surface and buried receivers."""
+    an interior moment source, surface and buried receivers.  fscale
+    multiplies every source frequency (and divides the source times)."""
     rng = np.random.default_rng(seed)
     x, y, z = spherical_grid(ni, nj, nk, h, colat0=colat0, r_top=r_top)
     nx, ny, nz = ni + 6, nj + 6, nk + 6
@@ -242,15 +243,16 @@ def small_test(ni=40, nj=36, nk=32, nt=120, h=1000.0, seed=7, freenode=True, pml
     nab = (pml, pml, pml, pml, pml, 0 if freenode else pml)
     m = Model("small_test", ni, nj, nk, dt, nt, freenode, x, y, z,
               np.asfortranarray(lam), np.asfortranarray(mu), np.asfortranarray(rho),
-              abs_number=nab, abs_velocity=(6500.0,) * 6, cfs_bmax=(2.5,) * 6, cfs_amax=(np.pi * 0.6,) * 6)
+              abs_number=nab, abs_velocity=(6500.0,) * 6, cfs_bmax=(2.5,) * 6,
+              cfs_amax=(np.pi * 0.6 * fscale,) * 6)
     nk2 = LENFD + nk - 1
     f = Sources(n=2, ntw=2, stf_id=STF_RICKER)
     f.indx = np.array([[LENFD + ni // 2 + 1, LENFD + nj // 2 + 1, nk2 + 1],
                        [LENFD + ni // 3 + 1, LENFD + 2 * nj // 3 + 1, nk2 - 1 + 1]], np.int32)
     f.shift = np.array([[0.0, 0.0, 0.0], [0.3, -0.2, 0.1]], np.float32)
-    f.t0 = np.array([0.0, 0.3], np.float32)
-    f.stf_time = np.array([1.6, 2.2], np.float32)
-    f.stf_freq = np.array([0.7, 0.5], np.float32)
+    f.t0 = np.array([0.0, 0.3 / fscale], np.float32)
+    f.stf_time = np.array([1.6 / fscale, 2.2 / fscale], np.float32)
+    f.stf_freq = np.array([0.7 * fscale, 0.5 * fscale], np.float32)
     f.comps = (rng.standard_normal((3, 4)) * 1e15).astype(np.float32)
     f.comps[2] += 2e15
     m.force = f
@@ -258,9 +260,9 @@ def small_test(ni=40, nj=36, nk=32, nt=120, h=1000.0, seed=7, freenode=True, pml
         mo = Sources(n=1, ntw=1, stf_id=STF_RICKER)
         mo.indx = np.array([[LENFD + 2 * ni // 3 + 1, LENFD + nj // 3 + 1, LENFD + nk // 2 + 1]], np.int32)
         mo.shift = np.array([[0.1, 0.2, -0.3]], np.float32)
-        mo.t0 = np.array([0.2], np.float32)
-        mo.stf_time = np.array([1.5], np.float32)
-        mo.stf_freq = np.array([0.6], np.float32)
+        mo.t0 = np.array([0.2 / fscale], np.float32)
+        mo.stf_time = np.array([1.5 / fscale], np.float32)
+        mo.stf_freq = np.array([0.6 * fscale], np.float32)
         mo.comps = (rng.standard_normal((6, 1)) * 1e15).astype(np.float32)
         m.moment = mo
     ri = rng.integers(LENFD, LENFD + ni, size=8)
@@ -269,3 +271,20 @@ def small_test(ni=40, nj=36, nk=32, nt=120, h=1000.0, seed=7, freenode=True, pml
     m.rec_indx = np.stack([ri + 1, rj + 1, rk + 1], 1).astype(np.int32)
     m.rec_tinv = 1
     return m
+
+
+def curved_test(ni=40, nj=36, nk=24, nt=96, seed=41, colat0=1.0, r_top=20000.0, h=200.0, **kw):
+    """small_test on a strongly curved grid away from the equator: colatitude
+    centred on colat0 (cot = 0.64, 1/sin = 1.19 at 1 rad), top radius r_top
+    (20 km: h / r = 1e-2 per cell), spacing h.  The time step keeps the
+    Courant number of small_test (0.32 on the smallest arc spacing, at the
+    bottom of the grid), and the sources are scaled to the spacing.  On this
+    grid the cot(theta), 1/sin(theta) and 1/r terms of the equations are of
+    the order of the derivative terms."""
+    x, _, z = spherical_grid(ni, nj, nk, h, colat0=colat0, r_top=r_top)
+    dth = h / r_top
+    dmin = min(h, z[LENFD] * dth * float(np.min(np.sin(x))))
+    vmax = 1.75 * 3400.0 * 1.08
+    dt = float(np.float32(0.32 * dmin / vmax))
+    return small_test(ni=ni, nj=nj, nk=nk, nt=nt, h=h, seed=seed, colat0=colat0, r_top=r_top, dt=dt,
+                      fscale=1000.0 / h * 0.8, **kw)

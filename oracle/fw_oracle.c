@@ -129,6 +129,20 @@ typedef struct {
 
 #define IDX(s, i, j, k) ((long)(i) + (s)->sy * (long)(j) + (s)->sz * (long)(k))
 
+/* Planted mistakes, for the control tests only (0: none).  A control test
+ * switches one on, shows that the check it guards fails, and switches it off:
+ *   1  sign of the cot(theta) terms of hVx flipped
+ *   2  1/sin(theta) of the y derivative in hVx replaced by sin(theta)
+ *   3  sign of the cot(theta) term of the y-face PML grouping of hVx flipped
+ *   4  sign of the cot(theta) term of E22 flipped
+ *   5  source times shifted by half an RK stage (tinc + 0.25)
+ *   6  smoothed body force also applied on the free-surface plane
+ *   7  smoothed body force doubled
+ *   8  surface renormalisation of the smoothing weights skipped
+ *   9  surface traction of a surface force doubled */
+static int g_mut = 0;
+void ORC(set_mutation)(int m) { g_mut = m; }
+
 static real *zalloc(long n) {
   real *p = (real *)calloc((size_t)(n > 0 ? n : 1), sizeof(real));
   if (!p) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
@@ -608,10 +622,13 @@ static void op_sweep(orc_solver *s, int xf, int yf, int zf) {
         real Txx = W[TXX][c], Tyy = W[TYY][c], Tzz = W[TZZ][c], Txy = W[TXY][c], Txz = W[TXZ][c],
              Tyz = W[TYZ][c], Vx = W[VX][c], Vy = W[VY][c], Vz = W[VZ][c];
         h[VX][c] = rrho * (DxTxx / zk + DyTxy / zk / xs + DzTxz + (C_(3.0) * Txz + Txx * xc - Tyy * xc) / zk);
+        if (g_mut == 1) h[VX][c] = rrho * (DxTxx / zk + DyTxy / zk / xs + DzTxz + (C_(3.0) * Txz - Txx * xc + Tyy * xc) / zk);
+        if (g_mut == 2) h[VX][c] = rrho * (DxTxx / zk + DyTxy / zk * xs + DzTxz + (C_(3.0) * Txz + Txx * xc - Tyy * xc) / zk);
         h[VY][c] = rrho * (DxTxy / zk + DyTyy / zk / xs + DzTyz + (C_(2.0) * Txy * xc + C_(3.0) * Tyz) / zk);
         h[VZ][c] = rrho * (DxTxz / zk + DyTyz / zk / xs + DzTzz + (C_(2.0) * Tzz - Txx - Tyy + Txz * xc) / zk);
         real E11 = (DxVx + Vz) / zk;
         real E22 = (Vx * xc + DyVy / xs + Vz) / zk;
+        if (g_mut == 4) E22 = (-Vx * xc + DyVy / xs + Vz) / zk;
         real E12 = (DxVy + DyVx / xs - Vy * xc) / zk / C_(2.0);
         real E33 = DzVz;
         real E13 = (DxVz / zk + DzVx - Vx / zk) / C_(2.0);
@@ -716,7 +733,8 @@ static void abs_correct(orc_solver *s, int xf, int yf, int zf) {
             b1 = C_(1.0) / s->By[j]; b1_1 = b1 - C_(1.0); ad = s->Ay[j] + s->Dy[j]; Dd = s->Dy[j];
             real et = s->eta_y[j];
             real DyTxy = opD(yf, W[TXY] + c, sy) * et;
-            DyTxy = (DyTxy / xs + (W[TXX][c] - W[TYY][c]) * xc + W[TXZ][c]) / zk;
+            DyTxy = g_mut == 3 ? (DyTxy / xs - (W[TXX][c] - W[TYY][c]) * xc + W[TXZ][c]) / zk
+                                : (DyTxy / xs + (W[TXX][c] - W[TYY][c]) * xc + W[TXZ][c]) / zk;
             h[VX][c] = h[VX][c] + (b1_1 * DyTxy - b1 * ps[4][a]) * rrho;
             ph[4][a] = Dd * DyTxy - ad * ps[4][a];
             real DyTyy = opD(yf, W[TYY] + c, sy) * et;
@@ -783,7 +801,7 @@ static real cell_volume(orc_solver *s, int i, int j, int k) {
          (s->z[k] * s->z[k]) * s->xsin[i];
 }
 static void renorm_surface(orc_solver *s, real *normd, int sk) {
-  if (s->freenode && sk + LENFD > s->nk2) {
+  if (s->freenode && sk + LENFD > s->nk2 && g_mut != 8) {
     real sum = 0;
     int kmax = s->nk2 - sk;
     for (int k = -LENFD; k <= kmax; k++)
@@ -804,6 +822,7 @@ static void src_surface(orc_solver *s, int ntime, real tinc) {
       if (i <= s->ni2 && i >= s->ni1 && j <= s->nj2 && j >= s->nj1 && k == s->nk2) {
         real S = (s->x[i + 1] - s->x[i - 1]) * (s->y[j + 1] - s->y[j - 1]) / C_(4.0) * (s->z[k] * s->z[k]) * s->xsin[i];
         long c2 = i + s->sy * j;
+        if (g_mut == 9) S = S / C_(2.0);
         s->TxSrc[c2] = fx0 / S * disp; s->TySrc[c2] = fy0 / S * disp; s->TzSrc[c2] = fz0 / S * disp;
         s->VxSrc[c2] = fx0 / S * rate; s->VySrc[c2] = fy0 / S * rate; s->VzSrc[c2] = fz0 / S * rate;
       }
@@ -832,7 +851,8 @@ static void src_force(orc_solver *s, int ntime, real tinc) {
               real V = cell_volume(s, i, j, k);
               V = C_(1.0) / (V * s->rho[c]) * d;
               real fx = fx0 * V, fy = fy0 * V, fz = fz0 * V;
-              if (s->freenode && k == s->nk2) continue;
+              if (s->freenode && k == s->nk2 && g_mut != 6) continue;
+              if (g_mut == 7) { fx *= C_(2.0); fy *= C_(2.0); fz *= C_(2.0); }
               s->h[VX][c] = s->h[VX][c] + fx; s->h[VY][c] = s->h[VY][c] + fy; s->h[VZ][c] = s->h[VZ][c] + fz;
             }
           }
@@ -887,6 +907,7 @@ void ORC(step)(orc_solver *s, int ntime) {
   abs_syn(s);
   for (int st = 0; st < 4; st++) {
     real tinc = st == 0 ? C_(0.0) : rka(st - 1);
+    if (g_mut == 5) tinc += C_(0.25);
     int pat[3];
     ORC(stage_pattern)(ntime - s->cycle_origin, st, pat);
     src_surface(s, ntime, tinc);

@@ -69,6 +69,7 @@ def _load(prec):
         "f32_to_bf16": (C.c_uint16, [C.c_float]),
         "f32_to_fp16": (C.c_uint16, [C.c_float]),
         "fp16_to_f32": (C.c_float, [C.c_uint16]),
+        "set_mutation": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, p + name)
@@ -126,6 +127,23 @@ def sgt_decode(fmt, codes, scales, subc):
     Lib(32).sgt_decode(f, int(subc[0]), int(subc[1]), int(subc[2]), codes.size // npts, codes.ctypes.data,
                        scales.ctypes.data, out.ctypes.data)
     return out
+
+
+class mutation:
+    """Context manager planting mistake `m` in the oracle (both precisions)
+    for a control test; see fw_oracle.c for the list."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        for prec in (32, 64):
+            Lib(prec).set_mutation(self.m)
+        return self
+
+    def __exit__(self, *a):
+        for prec in (32, 64):
+            Lib(prec).set_mutation(0)
 
 
 def build_metric(model):
