@@ -26,7 +26,7 @@ SYMBOLS = (
     "fw_checkpoint_size", "fw_checkpoint_save", "fw_checkpoint_load", "fw_get_wavefield",
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_scatter_param", "fwk_vscatter", "fwk_eoshift", "fwk_reparam",
-    "fwk_filter", "fwk_scatter2kernel", "fwk_rotate",
+    "fwk_filter", "fwk_scatter2kernel", "fwk_scatter2kernel_noalloc", "fwk_rotate",
     "fwk_last_error", "fwae_conv_fwd", "fwae_conv_bwd_data", "fwae_conv_bwd_weight", "fwae_act_fwd", "fwae_act_bwd",
     "fwae_mse", "fwae_adam", "fwae_blocks", "fwk_assemble", "fwl_create", "fwl_solve", "fwl_destroy", "fwl_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
     "fw_memcpy_d2d", "fw_device_sync",
@@ -109,6 +109,8 @@ def _sig(lib):
         "fwk_filter": (C.c_int, [C.c_int, C.c_long, C.c_int, C.c_int, dp, dp, vp, vp, vp]),
         "fwk_scatter2kernel": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, fp, fp,
                                          vp, vp, vp, vp, vp, vp, vp]),
+        "fwk_scatter2kernel_noalloc": (C.c_int, [C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, fp,
+                                                 fp, vp, vp, vp, vp, vp, vp, vp]),
         "fwk_rotate": (C.c_int, [C.c_long, vp, vp, vp, fp, vp]),
         "fwk_last_error": (C.c_char_p, []),
         "fwae_conv_fwd": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, vp, vp]),

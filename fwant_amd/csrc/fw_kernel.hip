@@ -7,6 +7,7 @@
 #include "fw_common.h"
 #include "../../include/fwant.h"
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -580,30 +581,52 @@ int fwk_filter(int mode, long npt, int knt, int nod1, const double* b, const dou
   FiltCoef cf{};
   for (int q = 0; q < nod1; ++q) { cf.b[q] = b[q]; cf.a[q] = a[q]; }
   const int NEL = knt + 6 * (nod1 - 1);
-  double* scratch = nullptr;
+  double* fwd = nullptr;
   hipStream_t st = (hipStream_t)stream;
   // filtfilt keeps the whole forward-pass output (NEL doubles per point)
-  if (mode == 2) KCHK(hipMallocAsync((void**)&scratch, sizeof(double) * (size_t)NEL * npt, st));
-  hipLaunchKernelGGL(k_filter, dim3(nblocks(npt, 256)), dim3(256), 0, st, mode, npt, knt, nod1, cf, w0, w, scratch);
+  if (mode == 2) KCHK(hipMallocAsync((void**)&fwd, sizeof(double) * (size_t)NEL * npt, st));
+  hipLaunchKernelGGL(k_filter, dim3(nblocks(npt, 256)), dim3(256), 0, st, mode, npt, knt, nod1, cf, w0, w, fwd);
   KCHK(hipGetLastError());
-  if (scratch) KCHK(hipFreeAsync(scratch, st));
+  if (fwd) KCHK(hipFreeAsync(fwd, st));
   return FW_OK;
 }
+
+}  // extern "C"
+
+// normalisation of SI_ker_sta.F90:1391-1397 (host, double): M0 * int V^2 dt
+// and M0 * int U^2 dt by the trapezoid rule over [nt1, nt2]
+static void s2k_norms(int nt1, int nt2, float dt, float M0, const float* V, const float* U, double* V0, double* U0) {
+  double v = (0.5 * (double)V[nt1 - 1] * V[nt1 - 1] + 0.5 * (double)V[nt2 - 1] * V[nt2 - 1]) * dt;
+  double u = (0.5 * (double)U[nt1 - 1] * U[nt1 - 1] + 0.5 * (double)U[nt2 - 1] * U[nt2 - 1]) * dt;
+  for (int n = nt1 + 1; n <= nt2 - 1; ++n) {
+    v += (double)V[n - 1] * V[n - 1] * dt;
+    u += (double)U[n - 1] * U[n - 1] * dt;
+  }
+  *V0 = v * M0;
+  *U0 = u * M0;
+}
+
+// Staging of V and U for the allocation-free scatter2kernel: one copy per
+// GPU, held until the process ends.  Host code never passes this name to a
+// kernel or a copy (it is a host-side stand-in there): it uses the device
+// address from hipGetSymbolAddress, fetched once per device.
+constexpr int S2K_MAX_KNT = 16384;
+static __device__ __align__(256) float scratch[2 * S2K_MAX_KNT];
+namespace {
+constexpr int S2K_MAX_DEV = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[S2K_MAX_DEV] = {};
+}  // namespace
+
+extern "C" {
 
 int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, const float* V, const float* U,
                        const float* Sa, const float* Sb, float* Kap, float* Kaq, float* Kbp, float* Kbq, void* stream) {
   if (npt <= 0 || knt <= 0 || nt1 < 1 || nt2 > knt || nt1 >= nt2 || !V || !U || !Sa || !Sb || !Kap || !Kaq || !Kbp ||
       !Kbq)
     return kfail(FW_ERR_ARG, "fwk_scatter2kernel: bad argument (1 <= nt1 < nt2 <= knt)");
-  // normalisation of SI_ker_sta.F90:1391-1397 (host, double)
-  double V0 = (0.5 * (double)V[nt1 - 1] * V[nt1 - 1] + 0.5 * (double)V[nt2 - 1] * V[nt2 - 1]) * dt;
-  double U0 = (0.5 * (double)U[nt1 - 1] * U[nt1 - 1] + 0.5 * (double)U[nt2 - 1] * U[nt2 - 1]) * dt;
-  for (int n = nt1 + 1; n <= nt2 - 1; ++n) {
-    V0 += (double)V[n - 1] * V[n - 1] * dt;
-    U0 += (double)U[n - 1] * U[n - 1] * dt;
-  }
-  V0 *= M0;
-  U0 *= M0;
+  double V0, U0;
+  s2k_norms(nt1, nt2, dt, M0, V, U, &V0, &U0);
   hipStream_t st = (hipStream_t)stream;
   float* dVU = nullptr;
   KCHK(hipMallocAsync((void**)&dVU, sizeof(float) * 2 * knt, st));
@@ -613,6 +636,44 @@ int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, 
                      dVU + knt, Sa, Sb, Kap, Kaq, Kbp, Kbq);
   KCHK(hipGetLastError());
   KCHK(hipFreeAsync(dVU, st));
+  KCHK(hipStreamSynchronize(st));
+  return FW_OK;
+}
+
+int fwk_scatter2kernel_noalloc(long npt, int knt, int nt1, int nt2, float dt, float M0, const float* V,
+                               const float* U, const float* Sa, const float* Sb, float* Kap, float* Kaq, float* Kbp,
+                               float* Kbq, void* stream) {
+  if (npt <= 0 || knt <= 0 || nt1 < 1 || nt2 > knt || nt1 >= nt2 || !V || !U || !Sa || !Sb || !Kap || !Kaq || !Kbp ||
+      !Kbq)
+    return kfail(FW_ERR_ARG, "fwk_scatter2kernel_noalloc: bad argument (1 <= nt1 < nt2 <= knt)");
+  if (knt > S2K_MAX_KNT)
+    return kfail(FW_ERR_ARG, "fwk_scatter2kernel_noalloc: knt = " + std::to_string(knt) +
+                                 " exceeds the staging size of 16384 samples (use fwk_scatter2kernel)");
+  int dev = -1;
+  KCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= S2K_MAX_DEV) return kfail(FW_ERR_ARG, "fwk_scatter2kernel_noalloc: device index above 63");
+  float* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+      void* p = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !p)
+        return kfail(FW_ERR_HIP, std::string("fwk_scatter2kernel_noalloc: hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+      g_scratch_ptr[dev] = (float*)p;
+    }
+    ptr = g_scratch_ptr[dev];
+  }
+  double V0, U0;
+  s2k_norms(nt1, nt2, dt, M0, V, U, &V0, &U0);
+  hipStream_t st = (hipStream_t)stream;
+  // the copies overwrite every staged value the kernel reads (V, U: knt each)
+  KCHK(hipMemcpyAsync(ptr, V, sizeof(float) * knt, hipMemcpyHostToDevice, st));
+  KCHK(hipMemcpyAsync(ptr + knt, U, sizeof(float) * knt, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_kernel, dim3(nblocks(npt, 256)), dim3(256), 0, st, npt, nt1, nt2, dt, (float)V0, (float)U0, ptr,
+                     ptr + knt, Sa, Sb, Kap, Kaq, Kbp, Kbq);
+  KCHK(hipGetLastError());
   KCHK(hipStreamSynchronize(st));
   return FW_OK;
 }

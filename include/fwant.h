@@ -243,6 +243,17 @@ int fwk_filter(int mode, long npt, int knt, int nod1, const double *b, const dou
 int fwk_scatter2kernel(long npt, int knt, int nt1, int nt2, float dt, float M0, const float *V,
                        const float *U, const float *Sa, const float *Sb, float *Kap, float *Kaq, float *Kbp,
                        float *Kbq, void *stream);
+/* Allocation-free variant of fwk_scatter2kernel, same arguments and results
+ * (bit for bit).  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  V and U are staged in a fixed device
+ * array of 2 x 16384 floats per GPU (held until the process ends) instead of
+ * a per-call allocation; knt above 16384 is rejected with FW_ERR_ARG.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (from
+ * several host threads) race.  Returns after the kernel has finished. */
+int fwk_scatter2kernel_noalloc(long npt, int knt, int nt1, int nt2, float dt, float M0, const float *V,
+                               const float *U, const float *Sa, const float *Sb, float *Kap, float *Kaq,
+                               float *Kbp, float *Kbq, void *stream);
 /* scatter_rotate, SI_ker_sta.F90:1354-1373 (rot: HOST 3x3 column-major). In place. */
 int fwk_rotate(long n, float *X, float *Y, float *Z, const float *rot, void *stream);
 /* Restore packed SGT records (fw_add_snapshot_packed layout) to FP32:
