@@ -9,16 +9,18 @@ resident in HBM before the timed region starts.
 
 Prints ONE JSON line:
   value      cell-updates/s (interior cells x steps / wall time of K steps)
-  roofline   one RK stage (k_vel then k_str over all cells, the dominant
-             work): algorithmic bytes per stage (156 B per cell, the average
-             of 120/192/192/120 B over the 4 RK stages) over its mean
-             duration measured with HIP events on the solver's stream,
+  roofline   one RK stage (k_stage over all cells, the dominant work):
+             algorithmic bytes per stage (156 B per cell, the average of
+             120/192/192/120 B over the 4 RK stages of the 3-register RK4
+             form) over its mean duration measured with HIP events on the
+             solver's stream,
              against the 8 TB/s HBM peak; traffic = HBM bytes per stage
              from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same
              counters on a plain copy (summarised by --summarize-pmc into
              profiles/stage_traffic_gradient_400x400x200.json)
-  cpu_baseline  the FP32 C oracle (OpenMP) on the host cores, timed on a
-             bounded sample of the same workload (2 full steps)
+  cpu_baseline  the FP32 C oracle (OpenMP) on every host core this
+             process may run on, timed on a bounded sample of the same
+             workload (2 full steps); nproc and the CPU model reported
   accuracy   receiver records of the two-layer 100x100x50 workload (1008
              steps) against the committed FP32 / FP64 CPU records
   pipeline   station-pair kernels/hour, plain (one FP32 solve per station)
@@ -62,7 +64,7 @@ def _pmc_means(db):
 
 
 def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, out):
-    """HBM bytes per RK stage (k_vel + k_str) from FETCH_SIZE / WRITE_SIZE runs.
+    """HBM bytes per RK stage (k_stage) from FETCH_SIZE / WRITE_SIZE runs.
     Each counter is scaled by bytes-moved / counter-value of a plain 1 GiB device copy."""
     f, nf = _pmc_means(fetch_db)
     w, nw = _pmc_means(write_db)
@@ -71,11 +73,11 @@ def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, out):
     ff = COPY_BYTES / (cf["copy"] * 1024.0)
     fw = COPY_BYTES / (cw["copy"] * 1024.0)
     kern = {}
-    for k in ("k_vel", "k_str"):
+    for k in ("k_stage",):
         kern[k] = {"fetch_bytes": f[k] * 1024.0 * ff, "write_bytes": w[k] * 1024.0 * fw,
                    "dispatches": nf[k]}
     total = sum(v["fetch_bytes"] + v["write_bytes"] for v in kern.values())
-    res = {"bytes_per_launch": total, "launch": "one RK stage = k_vel + k_str over all cells",
+    res = {"bytes_per_launch": total, "launch": "one RK stage = k_stage over all cells",
            "fetch_factor": ff, "write_factor": fw, "kernels": kern}
     with open(out, "w") as fh:
         json.dump(res, fh, indent=1)
@@ -83,7 +85,7 @@ def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, out):
 
 
 def summarize_trace(db, ncell):
-    """Markdown table of kernel durations by family (and by RK kind for the stage kernels)."""
+    """Markdown table of kernel durations by family (and by RK form for the stage kernel)."""
     import re
     import sqlite3
     con = sqlite3.connect(db)
@@ -91,20 +93,32 @@ def summarize_trace(db, ncell):
     acc = {}
     for name, d in rows:
         key = _kernel_family(name)
-        m = re.search(r"(k_vel|k_str)<([^>]*)>", name)
+        m = re.search(r"(k_stage)<([^>]*)>", name)
         if m:
-            key += f" kind {m.group(2).split(',')[3].strip()}"
+            key += f" form {m.group(2).split(',')[3].strip()}"
         acc.setdefault(key, []).append(d / 1000.0)
     tot = sum(sum(v) for v in acc.values())
     print("| kernel | calls | total us | mean us | share |\n|---|---|---|---|---|")
     for k, v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
         print(f"| {k} | {len(v)} | {sum(v):.0f} | {sum(v) / len(v):.1f} | {100 * sum(v) / tot:.1f}% |")
-    st = [sum(acc.get(k, [])) for k in acc if k.startswith("k_vel") or k.startswith("k_str")]
-    nst = len([1 for k, v in acc.items() if k.startswith("k_vel") for _ in v])
+    st = [sum(acc.get(k, [])) for k in acc if k.startswith("k_stage")]
+    nst = len([1 for k, v in acc.items() if k.startswith("k_stage") for _ in v])
     if nst:
         mean = sum(st) / nst
-        print(f"\nmean stage (k_vel + k_str) {mean:.1f} us -> "
+        print(f"\nmean stage (k_stage) {mean:.1f} us -> "
               f"{ncell * BYTES_PER_CELL_LAUNCH / (mean * 1e-6) / 1e9:.0f} GB/s algorithmic")
+
+
+def cpu_model():
+    """CPU model string of the host (first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -251,25 +265,28 @@ def main():
     s.timing(False)
     avg_ms = stage_ms / max(nst, 1)
     achieved = ncell * BYTES_PER_CELL_LAUNCH / (avg_ms * 1e-3) / 1e9
-    log(f"RK stage (k_vel + k_str): {nst} stages, mean {avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic; "
+    log(f"RK stage (k_stage): {nst} stages, mean {avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic; "
         f"stage kernels {stage_ms / total_ms * 100:.1f}% of step time")
     traffic = None
     if args.traffic and os.path.exists(args.traffic) and (args.ni, args.nj, args.nk) == (400, 400, 200):
         with open(args.traffic) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        # only a summary of the current stage kernel counts
+        if "k_stage" in tj.get("kernels", {}):
+            traffic = tj.get("bytes_per_launch")
     s.close()
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle.pyoracle import OracleSolver
-        cores = min(16, os.cpu_count() or 1)
+        cores = len(os.sched_getaffinity(0))
         o = OracleSolver(m, 32, nthreads=cores)
         o.run(1)  # first-touch warm-up outside the timed sample
         t0 = time.perf_counter()
         o.run(args.cpu_steps)
         cel = time.perf_counter() - t0
         cpu = {"value": ncell * args.cpu_steps / cel, "unit": "cell-updates/s", "cores": cores,
-               "kind": "port",
+               "kind": "port", "nproc": os.cpu_count(), "cpu_model": cpu_model(),
                "sample": f"FP32 C oracle, {args.cpu_steps} full steps of {m.ni}x{m.nj}x{m.nk} "
                          f"after 1 warm-up step ({cel:.1f}s)"}
         log(f"cpu baseline: {cpu['value'] / 1e6:.2f} M cell-updates/s on {cores} threads")
@@ -297,7 +314,8 @@ def main():
                    "hbm_fraction_whole_step": value * BYTES_PER_CELL_UPDATE / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "RK stage = k_vel + k_str (operator sweep fused with the RK update)",
+                     "kernel": "RK stage = k_stage (operator sweep, CFS-PML and RK update; velocity and "
+                               "stress halves as two wave groups of one block)",
                      "mean_launch_us": avg_ms * 1e3},
         "cpu_baseline": cpu,
         "pipeline": pipe,

@@ -2,7 +2,8 @@
 
 The library is compiled from fwant_amd/csrc/*.hip with hipcc for gfx950 only,
 one object per translation unit, in parallel: the stage kernels
-(fw_stage_inst.hip) are compiled once per (kernel, tile shape).  Objects go
+(fw_stage_inst.hip) are compiled once per (tile shape, half of the F/B
+patterns).  Objects go
 to fwant_amd/_build/, the library to fwant_amd/libfwant.so.
 Packed-FP32 instruction selection is disabled (-packed-fp32-ops).
 """
@@ -28,6 +29,11 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
     "-Wno-unused-value",
+    # the stage kernel reads its 624-byte argument struct through several
+    # hundred field accesses; above InstCombine's default limit of users the
+    # struct is copied to scratch instead of read from the kernel-argument
+    # segment
+    "-mllvm", "-instcombine-max-copied-from-constant-users=4000",
 ]
 
 
@@ -61,10 +67,10 @@ def build_fortran(verbose: bool = True) -> str:
 def _units():
     """(object path, source path, extra flags) of every translation unit."""
     u = [(os.path.join(BUILD, s.replace(".hip", ".o")), os.path.join(CSRC, s), []) for s in SOURCES]
-    for k, name in ((0, "vel"), (1, "str")):
-        for t in range(NTILE):
-            u.append((os.path.join(BUILD, f"fw_stage_{name}{t}.o"), os.path.join(CSRC, STAGE_SRC),
-                      [f"-DFW_KERNEL={k}", f"-DFW_TILE={t}"]))
+    for t in range(NTILE):
+        for half in range(2):
+            u.append((os.path.join(BUILD, f"fw_stage_{t}_{half}.o"), os.path.join(CSRC, STAGE_SRC),
+                      [f"-DFW_TILE={t}", f"-DFW_PHALF={half}"]))
     return u
 
 

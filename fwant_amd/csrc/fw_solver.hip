@@ -6,20 +6,19 @@
 //   column-major ghosted volume (i fastest, row pitch px = nx rounded up to
 //   16 floats, interior column ni1 aligned to 64 bytes).
 //   A holds W_n (and doubles as the RK "m" register); B and C ping-pong the
-//   stage states; T is the RK accumulator.  Per step:
-//     stage 1: A -> B, T = A + b1 h      stage 2: B -> C (m = A), T += b2 h
-//     stage 3: C -> B (m = A), T += b3 h stage 4: B -> A = T + b4 h
-//   which is the minimal-traffic form of macdrp_syn/RK_beg/inn/fin
-//   (code/srcF/mod_macdrp.F90:268-393).
+//   stage states; T is the RK accumulator.  Per step (RK forms of
+//   fw_stage.h; r = 1/3):
+//     stage 1: A -> B                      stage 2: B -> C, T = A + r (B - A) + b2 h
+//     stage 3: C -> B                      stage 4: B -> A = T + r (B - A) + b4 h
+//   which is classical RK4 as macdrp_syn/RK_beg/inn/fin compute it
+//   (code/srcF/mod_macdrp.F90:268-393) with the accumulator written once.
 //   PML auxiliary variables use the same A/B/C/T rotation per face slab.
 //
 // Kernels per RK stage:
 //   k_surface   free-surface stress imaging (mod_macdrp.F90:3414-3436),
-//               saving the pre-image surface stresses at stage 1
-//   k_vel       velocity half of the operator sweep + CFS-PML (stress
-//               derivative auxiliaries) + RK update           (fw_stage.h)
-//   k_str       stress half of the operator sweep + CFS-PML (velocity
-//               derivative auxiliaries) + RK update           (fw_stage.h)
+//               saving the pre-image surface stresses the RK form reads
+//   k_stage     operator sweep + CFS-PML + RK update, velocity and stress
+//               halves as two wave groups of one block      (fw_stage.h)
 //               (mod_macdrp.F90:790-962, 291-393; mod_abs_npml.F90:1149-2123,
 //               738-1105)
 //   k_src       point-source terms (mod_src.F90:207-414), RK fix-up
@@ -52,8 +51,10 @@ thread_local std::string g_create_err;
 // ---------------------------------------------------------------------------
 // Free-surface stress imaging (free_timg, mod_macdrp.F90:3414-3436) on the
 // stage input, with the surface traction of surface force sources
-// (src_surface, mod_src.F90:373-414).  At stage 1 the pre-image surface
-// stresses are saved first: they are the RK "m" values at k = nk2.
+// (src_surface, mod_src.F90:373-414).  The pre-image surface stresses are
+// saved first where the RK form reads them: at stage 1 (save = 1, planes
+// 0-2: the "m" values at k = nk2) and at stages 2 and 4 (save = 2, planes
+// 3-5: the W_1 / W_3 values there).
 // ---------------------------------------------------------------------------
 struct SurfEntry {  // one surface force point (the last (n, m) writing it wins)
   int n, m;
@@ -71,7 +72,7 @@ struct SurfArgs {
   const float* __restrict__ F;      // Fx Fy Fz blocks [3][ntw*n]
   long fs, sy, sz, ps, fblk, bstride;
   const int* __restrict__ clk;  // device step clock: clk[0] = ntime
-  int ni1, ni2, nj1, nj2, nk2, ntw, stf_id, save;
+  int ni1, ni2, nj1, nj2, nk2, ntw, stf_id, save;  // save: 0 none, 1 planes 0-2, 2 planes 3-5
   float tinc, stept;
 };
 __global__ void k_surface(SurfArgs g) {
@@ -86,7 +87,7 @@ __global__ void k_surface(SurfArgs g) {
   float* Tzz = g.win + slot * g.bstride + TZZ * g.fs;
   float* Txz = g.win + slot * g.bstride + TXZ * g.fs;
   float* Tyz = g.win + slot * g.bstride + TYZ * g.fs;
-  float* msurf = g.msurf + slot * 3 * g.ps;
+  float* msurf = g.msurf + slot * 6 * g.ps + (g.save == 2 ? 3 * g.ps : 0);
   // every load before any store: the images above the surface and the
   // stresses below it are distinct cells, and issuing the loads together
   // takes one memory latency instead of one per image plane
@@ -159,7 +160,7 @@ struct SrcArgs {
   long fs, bstride;
   const int* __restrict__ rowslot;  // member slot of each affected cell
   const int* __restrict__ clk;      // device step clock: clk[0] = ntime
-  int ncell, nact, kind;
+  int ncell, nact, kind;  // kind: the RK form (stage index)
   float tinc, stept, a, b;
 };
 __global__ void k_src(SrcArgs g) {
@@ -211,22 +212,22 @@ __global__ void k_src(SrcArgs g) {
   }
   float* wout = g.wout + slot * g.bstride;
   float* t = g.t + slot * g.bstride;
-  // all loads, then all stores (wout and t are distinct register sets): one
-  // memory latency instead of one per field
+  // the RK form of the stage (fw_stage.h): forms 0 and 2 add a dh to the
+  // stage output only (the accumulator takes it through W_1 / W_3), form 1
+  // adds a dh and b dh, form 3 b dh.  All loads, then all stores (wout and
+  // t are distinct register sets): one memory latency instead of one per field
+  const bool upd_t = g.kind == 1;
+  const float cw = g.kind == 3 ? g.b : g.a;
   float wv[9], tv[9];
 #pragma unroll
   for (int f = 0; f < 9; ++f) {
     wv[f] = wout[f * g.fs + c];
-    tv[f] = g.kind == 2 ? 0.f : t[f * g.fs + c];
+    tv[f] = upd_t ? t[f * g.fs + c] : 0.f;
   }
 #pragma unroll
   for (int f = 0; f < 9; ++f) {
-    if (g.kind == 2) {
-      wout[f * g.fs + c] = wv[f] + g.b * dh[f];
-    } else {
-      wout[f * g.fs + c] = wv[f] + g.a * dh[f];
-      t[f * g.fs + c] = tv[f] + g.b * dh[f];
-    }
+    wout[f * g.fs + c] = wv[f] + cw * dh[f];
+    if (upd_t) t[f * g.fs + c] = tv[f] + g.b * dh[f];
   }
 }
 
@@ -986,7 +987,7 @@ static int build_sources(fw_handle h) {
 static int zero_state(fw_handle h) {
   const long n = (long)h->nbatch * h->bstride + h->off;
   hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_sets, n, 0.f);
-  const long ns = 3L * h->ps * h->nbatch;
+  const long ns = 6L * h->ps * h->nbatch;
   hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, h->stream, h->d_msurf, ns, 0.f);
   for (auto& f : h->face)
     if (f.on) {
@@ -1010,7 +1011,7 @@ static int alloc_state(fw_handle h) {
   h->d_sets = nullptr; h->d_msurf = nullptr; h->d_rec = nullptr;
   HIPCHK(hipMalloc(&h->d_sets, sizeof(float) * ((size_t)B * h->bstride + 64)));
   for (int q = 0; q < 4; ++q) h->set[q] = h->d_sets + h->off + (size_t)q * 9 * h->fs;
-  HIPCHK(hipMalloc(&h->d_msurf, sizeof(float) * 3 * (size_t)h->ps * B));
+  HIPCHK(hipMalloc(&h->d_msurf, sizeof(float) * 6 * (size_t)h->ps * B));
   if (h->npt > 0 && h->max_rec > 0) HIPCHK(hipMalloc(&h->d_rec, sizeof(float) * 9 * (size_t)h->npt * h->max_rec * B));
   h->nrec = 0;
   h->rec_cnt.assign(B, 0);
@@ -1126,16 +1127,13 @@ int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Kernel dispatch over the tile shapes, 8 F/B patterns and 3 RK kinds (the
+// Kernel dispatch over the tile shapes, 8 F/B patterns and 4 RK forms (the
 // instantiations live in fw_stage_inst.hip).
 // ---------------------------------------------------------------------------
-static StageFn vel_table(int c, int p, int kind) {
-  static StageFn (*const T[NTILECFG])(int, int) = {stage_fn_v0, stage_fn_v1, stage_fn_v2, stage_fn_v3};
-  return T[c](p, kind);
-}
-static StageFn str_table(int c, int p, int kind) {
-  static StageFn (*const T[NTILECFG])(int, int) = {stage_fn_s0, stage_fn_s1, stage_fn_s2, stage_fn_s3};
-  return T[c](p, kind);
+static StageFn stage_table(int c, int p, int form) {
+  static StageFn (*const T[NTILECFG][2])(int, int) = {{stage_fn_0_0, stage_fn_0_1}, {stage_fn_1_0, stage_fn_1_1},
+                                                       {stage_fn_2_0, stage_fn_2_1}, {stage_fn_3_0, stage_fn_3_1}};
+  return T[c][(p >> 2) & 1](p, form);
 }
 
 // Work items of the stage kernels: tiles over chunks of k planes in three
@@ -1175,11 +1173,13 @@ static int build_items(fw_handle h) {
   const int zhi = F[5].on ? F[5].lo[2] : h->nk2 + 1;  // first plane of the z2 slab
   const int nk = h->nk2 - h->nk1 + 1;
   const int ntile = (int)(xt.size() * yt.size() + xn.size() * ytn.size());
-  // items of about 5 planes (at least 4), 98304 / TY of them at 400x400x200
-  // (24576 for 4-row tiles): short items keep every XCD busy to the end of
-  // the launch, which outweighs the z-window restarts they add (measured
-  // over 700-36864 items and 4-16 planes per item)
-  const int target = 98304 / TY;
+  // items of about 14 planes, 32768 / TY of them at 400x400x200 (8192 for
+  // 4-row tiles): enough items to keep every XCD busy to the end of the
+  // launch, few enough that the z-window restarts at item starts stay small
+  // (the stage kernel at 400x400x200: 2048 items 1157 us, 4096 1139, 8192
+  // 1113, 12288 1125, 24576 1157, 49152 1186)
+  int target = 32768 / TY;
+  if (const char* ev = getenv("FWANT_ITEM_TARGET")) target = std::max(1, atoi(ev));
   int nch = std::max(1, std::min(std::max(1, nk / 4), (target + ntile - 1) / ntile));
   const int cs = std::max(1, (nk + nch - 1) / nch);
   std::vector<int4> narrow, lean, pml;
@@ -1203,10 +1203,14 @@ static int build_items(fw_handle h) {
         }
       }
     }
-  auto order = [](const int4& a, const int4& b) {
+  // items of a chunk in row-major (0) or column-major (1) tile order
+  int colmajor = 0;
+  if (const char* ev = getenv("FWANT_ITEM_ORDER")) colmajor = atoi(ev);
+  auto order = [colmajor](const int4& a, const int4& b) {
     if (a.z != b.z) return a.z < b.z;
-    if ((a.y & 0xffff) != (b.y & 0xffff)) return (a.y & 0xffff) < (b.y & 0xffff);
-    return (a.x & 0xffff) < (b.x & 0xffff);
+    const int ay = a.y & 0xffff, by = b.y & 0xffff, ax = a.x & 0xffff, bx = b.x & 0xffff;
+    if (colmajor) return ax != bx ? ax < bx : ay < by;
+    return ay != by ? ay < by : ax < bx;
   };
   std::stable_sort(narrow.begin(), narrow.end(), order);
   std::stable_sort(lean.begin(), lean.end(), order);
@@ -1215,6 +1219,7 @@ static int build_items(fw_handle h) {
   h->d_items = nullptr;
   h->n_narrow = (int)narrow.size();
   h->xg = std::max(1, ((int)xt.size() + 1) / 2);  // half a tile row (3 of 6 measured 1% faster than 1, 2, 4, 6, 12)
+  if (const char* ev = getenv("FWANT_XG")) h->xg = std::max(1, atoi(ev));
   h->n_lean = (int)lean.size();
   h->n_pml = (int)pml.size();
   std::vector<int4> all(narrow);
@@ -1234,7 +1239,6 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   const int flip = st & 1;
   const int xf = flip ? !bp[0] : bp[0], yf = flip ? !bp[1] : bp[1], zf = flip ? !bp[2] : bp[2];
   const int pat = (xf << 2) | (yf << 1) | zf;
-  const int kind = st == 0 ? 0 : (st == 3 ? 2 : 1);
   // register rotation: stage in/out sets
   static const int IN[4] = {0, 1, 2, 1}, OUT[4] = {1, 2, 1, 0};
   float* win = h->set[IN[st]];
@@ -1245,6 +1249,9 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   const float RKB[4] = {1.0f / 6.0f, 1.0f / 3.0f, 1.0f / 3.0f, 1.0f / 6.0f};
   const float tinc = st == 0 ? 0.f : RKA[st - 1];
   const float a = RKA[st] * h->cfg.stept, b = RKB[st] * h->cfg.stept;
+  // b_{s-1} / a_{s-1}: the share of W_s - m that the accumulator takes at
+  // the RK forms 1 and 3 (1/3 at both)
+  const float r = st == 1 || st == 3 ? RKB[st - 1] / RKA[st - 1] : 0.f;
 
   if (h->freenode) {
     SurfArgs s{};
@@ -1253,7 +1260,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
     s.fs = h->fs; s.sy = h->sy; s.sz = h->sz; s.ps = h->ps; s.fblk = (long)h->fntw * h->nforce;
     s.bstride = h->bstride; s.clk = h->d_clk;
     s.ni1 = h->ni1; s.ni2 = h->ni2; s.nj1 = h->nj1; s.nj2 = h->nj2; s.nk2 = h->nk2;
-    s.ntw = h->fntw; s.stf_id = h->fstf; s.save = st == 0; s.tinc = tinc; s.stept = h->cfg.stept;
+    s.ntw = h->fntw; s.stf_id = h->fstf; s.save = st == 0 ? 1 : (st == 2 ? 0 : 2); s.tinc = tinc; s.stept = h->cfg.stept;
     dim3 blk(64, 4), grd((h->cfg.ni + 63) / 64, (h->cfg.nj + 3) / 4, nact);
     hipLaunchKernelGGL(k_surface, grd, blk, 0, h->stream, s);
   }
@@ -1264,7 +1271,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
   g.msurf = h->d_msurf;
   g.fs = h->fs; g.sy = h->sy; g.sz = h->sz; g.ps = h->ps;
   g.ni1 = h->ni1; g.ni2 = h->ni2; g.nj1 = h->nj1; g.nj2 = h->nj2; g.nk1 = h->nk1; g.nk2 = h->nk2;
-  g.freenode = h->freenode; g.a = a; g.b = b;
+  g.freenode = h->freenode; g.a = a; g.b = b; g.r = r;
   g.nx = h->nx; g.ny = h->ny; g.fluidcol = h->d_fluidcol;
   g.n1 = h->nx + h->ny + h->nz;
   g.pb1 = h->d_pml; g.pad = h->d_pml + g.n1; g.pdd = h->d_pml + 2 * g.n1;
@@ -1281,7 +1288,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
     fa.ex0 = f.ex[0]; fa.ex1 = f.ex[1];
   }
   const int tc = h->tilecfg;
-  const dim3 sblk(TX, TILE_BYT[tc]);
+  const dim3 sblk(TX, 2 * TILE_BYT[tc]);
   if (h->timing) hipEventRecord(h->ev[2 * h->nstage], h->stream);
   g.items = h->d_items;
   g.nitems = h->n_narrow + h->n_pml + h->n_lean;
@@ -1294,8 +1301,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
     g.slot0 = s0;
     g.nm = std::min(W, nact - s0);
     const dim3 grid(g.nitems * g.nm);
-    hipLaunchKernelGGL(vel_table(tc, pat, kind), grid, sblk, 0, h->stream, g);
-    hipLaunchKernelGGL(str_table(tc, pat, kind), grid, sblk, 0, h->stream, g);
+    hipLaunchKernelGGL(stage_table(tc, pat, st), grid, sblk, 0, h->stream, g);
   }
   if (h->timing) {
     hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
@@ -1311,7 +1317,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact) {
     s.mt0 = h->d_mt0; s.mtime = h->d_mtime; s.mfreq = h->d_mfreq; s.M = h->d_M;
     s.mblk = (long)h->mntw * h->nmom; s.mntw = h->mntw; s.mstf = h->mstf;
     s.fs = h->fs; s.bstride = h->bstride; s.rowslot = h->d_srow; s.clk = h->d_clk;
-    s.ncell = h->n_srccell; s.nact = nact; s.kind = kind;
+    s.ncell = h->n_srccell; s.nact = nact; s.kind = st;
     s.tinc = tinc; s.stept = h->cfg.stept; s.a = a; s.b = b;
     hipLaunchKernelGGL(k_src, dim3((h->n_srccell + 127) / 128), dim3(128), 0, h->stream, s);
   }

@@ -1,27 +1,47 @@
-// Stage kernels of libfwant: the operator sweep Lx?_Ly?_Lz?
+// Stage kernel of libfwant: the operator sweep Lx?_Ly?_Lz?
 // (code/srcF/mod_macdrp.F90:790-962) fused with the RK update
-// (mod_macdrp.F90:291-393), split by output:
-//   k_vel  hVx hVy hVz from the six stresses (+ 1/rho)          -> 3 fields
-//   k_str  hTxx .. hTyz from the three velocities (+ lambda, mu) -> 6 fields
-// The RHS of a velocity depends on stresses only and vice versa, so the
-// split adds no traffic: every input field is read once per stage by one of
-// the two kernels.
+// (mod_macdrp.F90:291-393) and the CFS-PML corrections
+// (mod_abs_npml.F90:1149-2123, 738-1105), one launch per RK stage.
 //
-// Work decomposition (both kernels): a 256-thread block (4 waves) owns a
-// TW x TY tile of (i, j) columns over a chunk of k planes and marches k.
-// Tiles are TW = 64 columns wide (a wave row is one 256-byte row segment),
-// except the narrow tiles at the x edge of the volume (the remainder of the
-// x extent after whole 64-column tiles, when it is at most 32 columns),
-// which are TW = 16 columns wide: a wave then covers 4 rows of 16 columns,
-// and no lanes idle on a 16-column remainder.
-// The current k plane of every field that needs an x or y derivative is
-// staged in LDS with its one-sided halo (F: -1..+3, B: -3..+1), double
-// buffered so one barrier per plane suffices; z derivatives come from a
-// 5-slot register window per column.  While plane k is computed, the loads
-// of plane k+1 (window lead, halo) and the RK operands of plane k are in
-// flight.  In a 64-wide tile wave w owns rows j0 + w + 4q (q < RY): every
-// global access of a wave is 64 consecutive floats of one row.  Offsets are
-// 32-bit relative to per-field base pointers.
+// A block holds two wave groups on the same tile and planes:
+//   velocity group (waves 0 .. BYT-1)   hVx hVy hVz from the six stresses
+//   stress group   (waves BYT .. 2BYT-1) hTxx .. hTyz from the three velocities
+// The RHS of a velocity depends on stresses only and vice versa, so each
+// group keeps only its own stencil window in registers; the current plane of
+// each group's stencil fields is staged in LDS, and each group reads the
+// other group's plane centres from there: the stage input's own values at a
+// cell (the RK "m" of the first stage, and W_1 / W_3 of the RK form below)
+// cost no HBM reads.
+//
+// RK form.  The reference runs classical RK4 in a 4-register form (m = W_n;
+// stage s evaluates h_s on W_s; W_{s+1} = m + a_s h_s, t accumulates
+// b_s h_s; W_{n+1} = t + b_3 h_3).  Since b_0 = a_0 / 3 and b_2 = a_2 / 3
+// (a = 1/2, 1/2, 1; b = 1/6, 1/3, 1/3, 1/6), the accumulated b_0 h_0 and
+// b_2 h_2 equal (W_1 - m) / 3 and (W_3 - m) / 3, and the accumulator is
+// written at stage 1 only:
+//   form 0: W_1 = m + a_0 h_0                         (m = W_n = stage input)
+//   form 1: W_2 = m + a_1 h_1,  t = m + r (W_1 - m) + b_1 h_1
+//   form 2: W_3 = m + a_2 h_2
+//   form 3: W_{n+1} = t + r (W_3 - m) + b_3 h_3        (r = 1/3)
+// which moves 129 floats per cell and step instead of 156 (the same form
+// runs on the CFS-PML auxiliaries).  Source terms added after a stage
+// (k_src) add a_s dh to W_{s+1} only at forms 0 and 2, so they enter t
+// through the same differences.
+//
+// Work decomposition: a block owns a TW x TY tile of (i, j) columns over a
+// chunk of k planes and marches k.  Tiles are TW = 64 columns wide (a wave
+// row is one 256-byte row segment), except the narrow tiles at the x edge
+// of the volume (the remainder of the x extent after whole 64-column tiles,
+// when it is at most 32 columns), which are TW = 16 columns wide: a wave then
+// covers 4 rows of 16 columns, and no lanes idle on a 16-column remainder.
+// Planes in LDS carry their one-sided halo (F: -1..+3, B: -3..+1), double
+// buffered so one barrier per plane suffices (both groups meet at the same
+// barrier); z derivatives come from a 5-slot register window per column.
+// While plane k is computed, the loads of plane k+1 (window lead, halo) and
+// the RK operands of plane k are already in flight.  In a 64-wide tile wave
+// w of a group owns rows j0 + w + BYT q (q < RY): every global access of a
+// wave is 64 consecutive floats of one row.  Offsets are 32-bit relative to
+// per-field base pointers.
 #pragma once
 #include "fw_common.h"
 
@@ -43,6 +63,12 @@ __device__ __forceinline__ float lds_nt(const float* p) {
   return *p;
 #endif
 }
+// RK operands (m, t, media) of plane k+1 loaded while plane k is computed
+#ifndef FW_RK_AHEAD
+#define FW_RK_AHEAD 0
+#endif
+constexpr bool RK_AHEAD = FW_RK_AHEAD != 0;
+
 __device__ __forceinline__ void st_nt(float* p, float v) {
 #if FW_STREAM_NT
   __builtin_nontemporal_store(v, p);
@@ -62,10 +88,10 @@ struct FaceArgs {
 };
 
 struct StageArgs {
-  const float* __restrict__ win;
-  float* __restrict__ wout;
-  const float* __restrict__ m;
-  float* __restrict__ t;
+  const float* __restrict__ win;  // stage input W_s
+  float* __restrict__ wout;       // stage output (set A at form 3, where m is read through wout)
+  const float* __restrict__ m;    // set A = W_n
+  float* __restrict__ t;          // accumulator
   const float* __restrict__ lam;
   const float* __restrict__ mu;
   const float* __restrict__ rrho;
@@ -75,7 +101,10 @@ struct StageArgs {
   const float* __restrict__ rz;
   const float* __restrict__ rxsin;
   const float* __restrict__ xcot;
-  const float* __restrict__ msurf;       // pre-image Tzz, Txz, Tyz at k = nk2 (3 planes)
+  // surface copies at k = nk2 (Tzz, Txz, Tyz), 6 planes per member: 0-2 the
+  // pre-image stage-0 input (the "m" there), 3-5 the pre-image input of forms
+  // 1 and 3 (the W_1 / W_3 there)
+  const float* __restrict__ msurf;
   const int* __restrict__ fluidcol;      // 1 where the (i, j) column has a cell with mu <= SEIS_ZERO
   long fs, sy, sz, ps;
   int nx, ny;
@@ -83,7 +112,7 @@ struct StageArgs {
   int freenode;
   const int4* __restrict__ items;  // work items (i0 | ncol << 16, j0 | nrow << 16, k0, k1): a tile over a chunk of planes
   int nitems, p1, p2;  // item parts: narrow [0, p1), wide PML [p1, p2), wide interior
-  float a, b;
+  float a, b, r;       // a_s dt, b_s dt, r = b_{s-1} / a_{s-1}
   // CFS-PML: per-axis profiles 1/B, A + D/B, D/B (x: nx, y: ny, z: nz entries)
   const float* __restrict__ pb1;
   const float* __restrict__ pad;
@@ -92,8 +121,8 @@ struct StageArgs {
   FaceArgs face[6];
   // batched virtual sources: nm members from slot0 in one launch; a
   // member's register sets start bstride floats after the previous
-  // member's, its surface copy 3*ps after, its PML auxiliaries 24*cnt after
-  // (per face)
+  // member's, its surface copies 6*ps after, its PML auxiliaries 24*cnt
+  // after (per face)
   long bstride;
   int slot0, nm;
   int xg;  // items per XCD run (half the wide tiles of a row)
@@ -138,29 +167,34 @@ __device__ __forceinline__ float* aux_base(const FaceArgs& f, int slot) {
 
 // Auxiliary values of three variables of one face cell, loaded ahead of
 // use: psi of the stage's input set and, when FULL, the set-A and set-T
-// values of the update (by RK kind).
+// values the RK form reads.
 struct AuxVals {
   float p[3], a[3], t[3];
 };
 __device__ __forceinline__ float sel3(int m, float v0, float v1, float v2) { return m == 0 ? v0 : (m == 1 ? v1 : v2); }
 
-// auxiliary RK update of variable v (abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105);
-// pre: the set-A / set-T values were loaded ahead (pa, pt)
-template <int KIND, bool FULL>
+// auxiliary RK update of variable v in the RK form above (the reference's
+// abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105, in 4-register form); psi is
+// the stage input's value (W_1 / W_3 at forms 1 / 3).  pre: the set-A /
+// set-T values were loaded ahead (pa, pt)
+template <int FORM, bool FULL>
 __device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, int slot, long li, int v, float psi, float dpsi,
                                        bool pre, float pa, float pt) {
   const long S = 6 * f.cnt, q = v * f.cnt + li;
   float* const d = aux_base(f, slot);
-  if constexpr (KIND == 0) {
+  if constexpr (FORM == 0) {
     d[g.setout * S + q] = psi + g.a * dpsi;
-    d[3 * S + q] = psi + g.b * dpsi;
-  } else if constexpr (KIND == 1) {
-    const float A = FULL && pre ? pa : d[q], T = FULL && pre ? pt : d[3 * S + q];
-    d[g.setout * S + q] = A + g.a * dpsi;
-    d[3 * S + q] = T + g.b * dpsi;
   } else {
-    const float T = FULL && pre ? pt : d[3 * S + q];
-    d[g.setout * S + q] = T + g.b * dpsi;
+    const float A = FULL && pre ? pa : d[q];
+    if constexpr (FORM == 1) {
+      d[g.setout * S + q] = A + g.a * dpsi;
+      d[3 * S + q] = A + g.r * (psi - A) + g.b * dpsi;
+    } else if constexpr (FORM == 2) {
+      d[g.setout * S + q] = A + g.a * dpsi;
+    } else {
+      const float T = FULL && pre ? pt : d[3 * S + q];
+      d[g.setout * S + q] = T + g.r * (psi - A) + g.b * dpsi;
+    }
   }
 }
 __device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, int slot, long li, int v) {
@@ -200,26 +234,26 @@ __device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int slot, int i, i
 __device__ __forceinline__ int face_mask(const StageArgs& g, const AuxPre& a, int i, int j, int k) {
   return a.mask | (in_face(g.face[4], i, j, k) ? 16 : 0) | (in_face(g.face[5], i, j, k) ? 32 : 0);
 }
-template <int KIND, bool FULL>
+template <int FORM, bool FULL>
 __device__ __forceinline__ void aux_load(const AuxPre& a, int k, int k0, int setin, AuxVals& r) {
   const float* b = a.base + (long)(k - k0) * a.kst;
   const long c = a.cnt;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     r.p[m] = b[setin * 6 * c + m * c];
-    r.a[m] = FULL && KIND == 1 ? b[m * c] : 0.f;
-    r.t[m] = FULL && KIND != 0 ? b[18 * c + m * c] : 0.f;
+    r.a[m] = FULL && FORM != 0 ? b[m * c] : 0.f;
+    r.t[m] = FULL && FORM == 3 ? b[18 * c + m * c] : 0.f;
   }
 }
-// velocity half of the wide tiles: psi, set A and set T ahead; narrow tiles
-// and the stress half (more live RK operands): psi only
+// velocity group of the wide tiles: psi, set A and set T ahead; narrow
+// tiles and the stress group (more live RK operands): psi only
 constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = false;
 
 // CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
 // reference's face order x1 x2 y1 y2 z1 z2 (mod_abs_npml.F90:1149-2123, the
 // stress-derivative half).  D* are the raw one-sided derivatives times the
 // 1-D metric (before fluid zeroing).
-template <int KIND, bool FULL>
+template <int FORM, bool FULL>
 __device__ __forceinline__ void pml_vel(const StageArgs& g, int slot, int i, int j, int k, float h[3], float rr, float DxTxx,
                                         float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz, float DzTzz,
                                         float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz, float Txz,
@@ -254,16 +288,16 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int slot, int i, int
     h[0] += (b1_1 * D0 - b1 * q0) * rr;
     h[1] += (b1_1 * D1 - b1 * q1) * rr;
     h[2] += (b1_1 * D2 - b1 * q2) * rr;
-    aux_rk<KIND, FULL>(f, g, slot, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
-    aux_rk<KIND, FULL>(f, g, slot, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
-    aux_rk<KIND, FULL>(f, g, slot, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<FORM, FULL>(f, g, slot, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<FORM, FULL>(f, g, slot, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    aux_rk<FORM, FULL>(f, g, slot, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
   }
 }
 
 // CFS-PML corrections of hT (velocity-derivative half), including the
 // free-surface Hooke fix-ups of the x and y faces (mod_abs_npml.F90:1303-1333).
 // h: Txx Tyy Tzz Txy Txz Tyz.  DzV* are the standard 5-tap z derivatives.
-template <int KIND>
+template <int FORM>
 __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int j, int k, float h[6], float lam, float miu,
                                         float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
                                         float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
@@ -293,9 +327,9 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int
         h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
         h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
       }
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else if (ax == 1) {
       const float D0 = (DyVx * rxs - Vy * xc) * rzk;
       const float D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
@@ -309,18 +343,18 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int
         h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
         h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
       }
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else {
       h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
       h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
       h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
       h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
       h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<KIND, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
     }
   }
 }
@@ -360,11 +394,11 @@ __device__ __forceinline__ float zc(const float* w) {
   return FWD ? w[1] : w[3];
 }
 
-// Halo slot of one thread in one LDS plane: x halo (XL left + XR right
-// columns of the TY rows, threads tid < 4*TY) or y halo (YL + YR rows of the
-// TW columns, every thread).  Slots outside the volume or beyond what the
-// tile's computed columns ie / rows je read (partial tiles) hold zero and
-// never feed a computed cell.
+// Halo slot of one thread of a group in one LDS plane: x halo (XL left + XR
+// right columns of the TY rows, threads tid < 4*TY) or y halo (YL + YR rows
+// of the TW columns, every thread).  Slots outside the volume or beyond what
+// the tile's computed columns ie / rows je read (partial tiles) hold zero
+// and never feed a computed cell.
 struct HaloSlot {
   unsigned off;
   int lds;
@@ -399,29 +433,42 @@ __device__ __forceinline__ HaloSlot yhalo_slot(const StageArgs& g, int i0, int j
   h.lds = (row - j0 + 3) * LW + (c + 3);
   return h;
 }
+// Loads without branches: every lane's address is valid (lanes outside the
+// tile or the volume read column 0 of their plane, halo slots outside read
+// offset 0), and the value is selected afterwards.  The loads of a plane then
+// form one basic block that the compiler schedules as a whole, instead of one
+// exec-masked block per load.
+__device__ __forceinline__ float ldv(const float* __restrict__ p, unsigned o, bool ok) {
+  const float v = p[o];
+  return ok ? v : 0.f;
+}
 __device__ __forceinline__ float ld(const float* __restrict__ p, const HaloSlot& h, unsigned kz) {
-  return h.ok ? p[h.off + kz] : 0.f;
+  return ldv(p, h.off + kz, h.ok);
 }
 
+// LDS planes of the two groups (per buffer): velocity group Txx Txy Txz Tyy
+// Tyz (with halos) and Tzz (centres only); stress group Vx Vy Vz.
+template <int BYT, int RY, int TW>
+struct TileGeom {
+  static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH;
+};
+constexpr int VNL = 6, SNL = 3;
+enum { LTXX = 0, LTXY, LTXZ, LTYY, LTYZ, LTZZ };
 
 // ---------------------------------------------------------------------------
-// k_vel: momentum equations with the spherical geometric terms
+// Velocity group: momentum equations with the spherical geometric terms
 // (mod_macdrp.F90:928-933), fluid zeroing of shear-stress derivatives
-// (:917-926), RK update of Vx Vy Vz.
-// LDS: Txx (x), Txy (x, y), Txz (x), Tyy (y), Tyz (y); z window: Tzz Txz Tyz.
+// (:917-926), RK update of Vx Vy Vz.  z window: Tzz Txz Tyz.
+// own: this group's LDS (VNL planes x 2 buffers), other: the stress group's.
 // ---------------------------------------------------------------------------
-template <int BYT, int RY, int TW>
-struct VelLds {
-  static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, NL = 5, SIZE = 2 * NL * PL;
-};
-template <int XF, int YF, int ZF, int KIND, int BYT, int RY, int TW, bool PML>
-__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int slot, float* shbase) {
+template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML>
+__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
   // RS: rows per row group (the stride between a thread's rows)
-  constexpr int RS = BYT * (TX / TW), TY = RS * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, NL = 5;
+  using G = TileGeom<BYT, RY, TW>;
+  constexpr int RS = BYT * (TX / TW), TY = G::TY, LW = G::LW, PL = G::PL;
   // set-A / set-T values of the CFS-PML auxiliaries loaded ahead in wide tiles
   constexpr bool FULLV = AUX_FULL_VEL && TW == TX;
-  float(*sh)[NL * PL] = reinterpret_cast<float(*)[NL * PL]>(shbase);  // Txx Txy Txz Tyy Tyz, double buffered
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * threadIdx.y, tid = threadIdx.x + TX * threadIdx.y;
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * wy, tid = threadIdx.x + TX * wy;
   const int i0 = it.x & 0xffff, j0 = it.y & 0xffff, k0 = it.z, k1 = it.w;
   // last computed column and row: tiles end at CFS-PML slab boundaries
   const int ie = i0 + (it.x >> 16) - 1, je = j0 + (it.y >> 16) - 1;
@@ -432,7 +479,9 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
   const long bo = (long)slot * g.bstride;
   const float* __restrict__ W = g.win + bo;
   float* __restrict__ wout = g.wout + bo;
-  const float* __restrict__ mm = g.m + bo;
+  // form 3 writes set A in place: each cell's m is read, then overwritten,
+  // by the same thread through the one pointer
+  const float* __restrict__ mm = FORM == 3 ? wout : g.m + bo;
   float* __restrict__ tt = g.t + bo;
   const float* __restrict__ pTxx = W + TXX * fs;
   const float* __restrict__ pTyy = W + TYY * fs;
@@ -464,13 +513,13 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
       const int o = ZF ? s - 1 : s - 3;
-      w[q][0][s] = ok[q] ? pTzz[c0 + o * (int)sz] : 0.f;
-      w[q][1][s] = ok[q] ? pTxz[c0 + o * (int)sz] : 0.f;
-      w[q][2][s] = ok[q] ? pTyz[c0 + o * (int)sz] : 0.f;
+      w[q][0][s] = ldv(pTzz, c0 + o * (int)sz, ok[q]);
+      w[q][1][s] = ldv(pTxz, c0 + o * (int)sz, ok[q]);
+      w[q][2][s] = ldv(pTyz, c0 + o * (int)sz, ok[q]);
     }
-    cTxx[q] = ok[q] ? pTxx[c0] : 0.f;
-    cTyy[q] = ok[q] ? pTyy[c0] : 0.f;
-    cTxy[q] = ok[q] ? pTxy[c0] : 0.f;
+    cTxx[q] = ldv(pTxx, c0, ok[q]);
+    cTyy[q] = ldv(pTyy, c0, ok[q]);
+    cTxy[q] = ldv(pTxy, c0, ok[q]);
   }
   AuxPre pre[RY];
 #pragma unroll
@@ -478,14 +527,33 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
   unsigned kz = sz * (unsigned)k0;
   float hxTxx = ld(pTxx, hx, kz), hxTxy = ld(pTxy, hx, kz), hxTxz = ld(pTxz, hx, kz);
   float hyTxy = ld(pTxy, hy, kz), hyTyy = ld(pTyy, hy, kz), hyTyz = ld(pTyz, hy, kz);
+  // halo lanes without a slot write a private scratch word of their own
+  float* const spare = own + 2 * VNL * PL + (threadIdx.x & (TX - 1));
+  float nrr[RY], nmo[RY][3], nto[RY][3];
+  auto load_rk = [&](unsigned kp) {
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+      const unsigned c = cc[q] + kp;
+      nrr[q] = lds_nt(g.rrho + c);
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const long e = f * fs + c;
+        nmo[q][f] = FORM != 0 ? lds_nt(mm + e) : 0.f;
+        nto[q][f] = FORM == 3 ? lds_nt(tt + e) : 0.f;
+      }
+    }
+  };
+  if (RK_AHEAD) load_rk(kz);
 
   for (int k = k0; k <= k1; ++k, kz += sz) {
-    float* s = sh[(k - k0) & 1];
-    float* sTxx = s;
-    float* sTxy = s + PL;
-    float* sTxz = s + 2 * PL;
-    float* sTyy = s + 3 * PL;
-    float* sTyz = s + 4 * PL;
+    float* s = own + ((k - k0) & 1) * (VNL * PL);
+    const float* sv = other + ((k - k0) & 1) * (SNL * PL);  // Vx Vy Vz of the stage input
+    float* sTxx = s + LTXX * PL;
+    float* sTxy = s + LTXY * PL;
+    float* sTxz = s + LTXZ * PL;
+    float* sTyy = s + LTYY * PL;
+    float* sTyz = s + LTYZ * PL;
+    float* sTzz = s + LTZZ * PL;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const int o = (ty + RS * q + 3) * LW + tx + 3;
@@ -494,61 +562,65 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
       sTyy[o] = cTyy[q];
       sTxz[o] = zc<ZF>(w[q][1]);
       sTyz[o] = zc<ZF>(w[q][2]);
+      sTzz[o] = zc<ZF>(w[q][0]);
     }
-    if (hx.lds >= 0) {
-      sTxx[hx.lds] = hxTxx;
-      sTxy[hx.lds] = hxTxy;
-      sTxz[hx.lds] = hxTxz;
+    {
+      float* const px = hx.lds >= 0 ? s + hx.lds : spare;
+      const int dx = hx.lds >= 0 ? PL : 0;
+      px[LTXX * dx] = hxTxx;
+      px[LTXY * dx] = hxTxy;
+      px[LTXZ * dx] = hxTxz;
+      float* const py = hy.lds >= 0 ? s + hy.lds : spare;
+      const int dy = hy.lds >= 0 ? PL : 0;
+      py[LTXY * dy] = hyTxy;
+      py[LTYY * dy] = hyTyy;
+      py[LTYZ * dy] = hyTyz;
     }
-    if (hy.lds >= 0) {
-      sTxy[hy.lds] = hyTxy;
-      sTyy[hy.lds] = hyTyy;
-      sTyz[hy.lds] = hyTyz;
-    }
-    // RK operands of plane k
+    // RK operands of plane k (form 0: m is the stage input, read from the
+    // stress group's plane after the barrier); with RK_AHEAD those of plane
+    // k+1 are loaded here and plane k's come from the previous iteration
+    const bool more = k < k1;
+    const unsigned kn = more ? kz + sz : kz;
     float rr[RY], mo[RY][3], to[RY][3];
-    bool fl[RY];
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
-      const unsigned c = cc[q] + kz;
-      rr[q] = comp[q] ? lds_nt(g.rrho + c) : 0.f;
-      fl[q] = fluid[q] ? g.mu[c] <= SEIS_ZERO : false;
+      rr[q] = nrr[q];
 #pragma unroll
-      for (int f = 0; f < 3; ++f) {
-        const long e = f * fs + c;
-        mo[q][f] = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : mm + e) : 0.f;
-        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(tt + e) : 0.f;
+      for (int f = 0; f < 3; ++f) { mo[q][f] = nmo[q][f]; to[q][f] = nto[q][f]; }
+    }
+    if (RK_AHEAD) load_rk(kn); else load_rk(kz);
+    if (!RK_AHEAD) {
+#pragma unroll
+      for (int q = 0; q < RY; ++q) {
+        rr[q] = nrr[q];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) { mo[q][f] = nmo[q][f]; to[q][f] = nto[q][f]; }
       }
     }
-    // prefetch plane k+1
-    const bool more = k < k1;
+    // prefetch plane k+1 (the last plane re-reads plane k: every address valid)
     float nTxx[RY], nTyy[RY], nTxy[RY], lead[RY][3];
-    const unsigned kn = kz + sz;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const unsigned c = cc[q] + kn;
       const unsigned l = c + (ZF ? 3u : 1u) * sz;
-      const bool p = more && ok[q];
-      lead[q][0] = p ? pTzz[l] : 0.f;
-      lead[q][1] = p ? pTxz[l] : 0.f;
-      lead[q][2] = p ? pTyz[l] : 0.f;
-      nTxx[q] = p ? pTxx[c] : 0.f;
-      nTyy[q] = p ? pTyy[c] : 0.f;
-      nTxy[q] = p ? pTxy[c] : 0.f;
+      lead[q][0] = ldv(pTzz, l, ok[q]);
+      lead[q][1] = ldv(pTxz, l, ok[q]);
+      lead[q][2] = ldv(pTyz, l, ok[q]);
+      nTxx[q] = ldv(pTxx, c, ok[q]);
+      nTyy[q] = ldv(pTyy, c, ok[q]);
+      nTxy[q] = ldv(pTxy, c, ok[q]);
     }
-    float nhxTxx = 0.f, nhxTxy = 0.f, nhxTxz = 0.f, nhyTxy = 0.f, nhyTyy = 0.f, nhyTyz = 0.f;
-    if (more) {
-      nhxTxx = ld(pTxx, hx, kn); nhxTxy = ld(pTxy, hx, kn); nhxTxz = ld(pTxz, hx, kn);
-      nhyTxy = ld(pTxy, hy, kn); nhyTyy = ld(pTyy, hy, kn); nhyTyz = ld(pTyz, hy, kn);
-    }
+    const float nhxTxx = ld(pTxx, hx, kn), nhxTxy = ld(pTxy, hx, kn), nhxTxz = ld(pTxz, hx, kn);
+    const float nhyTxy = ld(pTxy, hy, kn), nhyTyy = ld(pTyy, hy, kn), nhyTyz = ld(pTyz, hy, kn);
     AuxVals pv[RY];
     if constexpr (PML) {
 #pragma unroll
       for (int q = 0; q < RY; ++q)
-        if (pre[q].face >= 0) aux_load<KIND, FULLV>(pre[q], k, k0, g.setin, pv[q]);
+        if (pre[q].face >= 0) aux_load<FORM, FULLV>(pre[q], k, k0, g.setin, pv[q]);
     }
-    __syncthreads();
+    // plane metric with the plane's other loads (used right after the barrier)
     const float ze = g.zeta_z[k], rzk = g.rz[k];
+    __syncthreads();
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
@@ -565,7 +637,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         float DzTxz = dz5<ZF>(w[q][1]) * ze;
         float DzTyz = dz5<ZF>(w[q][2]) * ze;
         const float rDxTxy = DxTxy, rDxTxz = DxTxz, rDyTxy = DyTxy, rDyTyz = DyTyz, rDzTxz = DzTxz, rDzTyz = DzTyz;
-        if (fl[q]) {
+        // fluid cells (mu <= SEIS_ZERO) occur only in the rare columns that hold one
+        if (fluid[q] && g.mu[c] <= SEIS_ZERO) {
           DxTxy = 0.f; DxTxz = 0.f; DyTxy = 0.f; DyTyz = 0.f; DzTxz = 0.f; DzTyz = 0.f;
         }
         const float Txx = cTxx[q], Tyy = cTyy[q], Txy = cTxy[q];
@@ -575,16 +648,21 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         h[1] = rr[q] * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
         h[2] = rr[q] * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
         if constexpr (PML)
-          if (pml_here) pml_vel<KIND, FULLV>(g, slot, i, j0 + ty + RS * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
+          if (pml_here) pml_vel<FORM, FULLV>(g, slot, i, j0 + ty + RS * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
                         rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
-          if constexpr (KIND == 2) {
-            st_nt(wout + e, to[q][f] + g.b * h[f]);
-          } else {
+          const float wc = sv[f * PL + o];  // the stage input's own value (velocities are never imaged)
+          if constexpr (FORM == 0) {
+            st_nt(wout + e, wc + g.a * h[f]);
+          } else if constexpr (FORM == 1) {
             st_nt(wout + e, mo[q][f] + g.a * h[f]);
-            st_nt(tt + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
+            st_nt(tt + e, mo[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
+          } else if constexpr (FORM == 2) {
+            st_nt(wout + e, mo[q][f] + g.a * h[f]);
+          } else {
+            st_nt(wout + e, to[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
           }
         }
       }
@@ -605,22 +683,22 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 }
 
 // ---------------------------------------------------------------------------
-// k_str: strain rates (mod_macdrp.F90:935-941), near-surface low-order z
-// operators (CondFreeVLOW, :872-915), the traction condition at k = nk2
+// Stress group: strain rates (mod_macdrp.F90:935-941), near-surface low-order
+// z operators (CondFreeVLOW, :872-915), the traction condition at k = nk2
 // with zero source terms (:942-950; k_src adds the source part), Hooke
-// (:952-957), RK update of the six stresses.  At k = nk2 the RK "m" values of
-// Tzz Txz Tyz are the pre-image stresses saved by k_surface.
-// LDS: Vx Vy Vz (x and y halos); z window: Vx Vy Vz.
+// (:952-957), RK update of the six stresses.  At k = nk2 the "m" values of
+// Tzz Txz Tyz are the pre-image stresses saved by k_surface at form 0, and
+// their W_1 / W_3 the pre-image inputs saved at forms 1 / 3.
+// z window: Vx Vy Vz.  own: this group's LDS (SNL planes x 2 buffers),
+// other: the velocity group's.
 // ---------------------------------------------------------------------------
-template <int BYT, int RY, int TW>
-struct StrLds {
-  static constexpr int TY = BYT * (TX / TW) * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH, SIZE = 2 * 3 * PL;
-};
-template <int XF, int YF, int ZF, int KIND, int BYT, int RY, int TW, bool PML>
-__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* shbase) {
-  constexpr int RS = BYT * (TX / TW), TY = RS * RY, LW = TW + 6, LH = TY + 6, PL = LW * LH;
-  float(*sh)[3 * PL] = reinterpret_cast<float(*)[3 * PL]>(shbase);  // Vx Vy Vz, double buffered
-  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * threadIdx.y, tid = threadIdx.x + TX * threadIdx.y;
+template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML>
+__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
+  using G = TileGeom<BYT, RY, TW>;
+  constexpr int RS = BYT * (TX / TW), TY = G::TY, LW = G::LW, PL = G::PL;
+  // LDS plane of each output field (Txx Tyy Tzz Txy Txz Tyz) in the velocity group's planes
+  constexpr int LP[6] = {LTXX, LTYY, LTZZ, LTXY, LTXZ, LTYZ};
+  const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * wy, tid = threadIdx.x + TX * wy;
   const int i0 = it.x & 0xffff, j0 = it.y & 0xffff, k0 = it.z, k1 = it.w;
   // last computed column and row: tiles end at CFS-PML slab boundaries
   const int ie = i0 + (it.x >> 16) - 1, je = j0 + (it.y >> 16) - 1;
@@ -631,8 +709,9 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
   const long bo = (long)slot * g.bstride;
   const float* __restrict__ W = g.win + bo;
   float* __restrict__ wout = g.wout + bo;
-  const float* __restrict__ mm = g.m + bo;
+  const float* __restrict__ mm = FORM == 3 ? wout : g.m + bo;  // as in the velocity group
   float* __restrict__ tt = g.t + bo;
+  const float* __restrict__ ms = g.msurf + (long)slot * 6 * g.ps;
   const float* __restrict__ pV0 = W + VX * fs;
   const float* __restrict__ pV1 = W + VY * fs;
   const float* __restrict__ pV2 = W + VZ * fs;
@@ -658,9 +737,9 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
       const int o = ZF ? s - 1 : s - 3;
-      w[q][0][s] = ok[q] ? pV0[c0 + o * (int)sz] : 0.f;
-      w[q][1][s] = ok[q] ? pV1[c0 + o * (int)sz] : 0.f;
-      w[q][2][s] = ok[q] ? pV2[c0 + o * (int)sz] : 0.f;
+      w[q][0][s] = ldv(pV0, c0 + o * (int)sz, ok[q]);
+      w[q][1][s] = ldv(pV1, c0 + o * (int)sz, ok[q]);
+      w[q][2][s] = ldv(pV2, c0 + o * (int)sz, ok[q]);
     }
   }
   AuxPre pre[RY];
@@ -669,9 +748,28 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
   unsigned kz = sz * (unsigned)k0;
   float hx0 = ld(pV0, hx, kz), hx1 = ld(pV1, hx, kz), hx2 = ld(pV2, hx, kz);
   float hy0 = ld(pV0, hy, kz), hy1 = ld(pV1, hy, kz), hy2 = ld(pV2, hy, kz);
+  // halo lanes without a slot write a private scratch word of their own
+  float* const spare = own + 2 * SNL * PL + (threadIdx.x & (TX - 1));
+  float nla[RY], nmu[RY], nmo[RY][6], nto[RY][6];
+  auto load_rk = [&](unsigned kp) {
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+      const unsigned c = cc[q] + kp;
+      nla[q] = lds_nt(g.lam + c);
+      nmu[q] = lds_nt(g.mu + c);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const long e = (TXX + f) * fs + c;
+        nmo[q][f] = FORM != 0 ? lds_nt(mm + e) : 0.f;
+        nto[q][f] = FORM == 3 ? lds_nt(tt + e) : 0.f;
+      }
+    }
+  };
+  if (RK_AHEAD) load_rk(kz);
 
   for (int k = k0; k <= k1; ++k, kz += sz) {
-    float* s = sh[(k - k0) & 1];
+    float* s = own + ((k - k0) & 1) * (SNL * PL);
+    const float* st = other + ((k - k0) & 1) * (VNL * PL);  // stresses of the stage input
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const int o = (ty + RS * q + 3) * LW + tx + 3;
@@ -679,59 +777,60 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
       s[PL + o] = zc<ZF>(w[q][1]);
       s[2 * PL + o] = zc<ZF>(w[q][2]);
     }
-    if (hx.lds >= 0) {
-      s[hx.lds] = hx0;
-      s[PL + hx.lds] = hx1;
-      s[2 * PL + hx.lds] = hx2;
-    }
-    if (hy.lds >= 0) {
-      s[hy.lds] = hy0;
-      s[PL + hy.lds] = hy1;
-      s[2 * PL + hy.lds] = hy2;
+    {
+      float* const px = hx.lds >= 0 ? s + hx.lds : spare;
+      const int dx = hx.lds >= 0 ? PL : 0;
+      px[0] = hx0;
+      px[dx] = hx1;
+      px[2 * dx] = hx2;
+      float* const py = hy.lds >= 0 ? s + hy.lds : spare;
+      const int dy = hy.lds >= 0 ? PL : 0;
+      py[0] = hy0;
+      py[dy] = hy1;
+      py[2 * dy] = hy2;
     }
     const bool surf = g.freenode && k == g.nk2;
-    // RK operands of plane k
+    // RK operands of plane k (form 0: m is the stage input, read from the
+    // velocity group's plane after the barrier; the surface copies of the
+    // imaged fields are read there too); with RK_AHEAD as in the velocity group
+    const bool more = k < k1;
+    const unsigned kn = more ? kz + sz : kz;
     float la[RY], mu_[RY], mo[RY][6], to[RY][6];
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
-      const unsigned c = cc[q] + kz;
-      la[q] = comp[q] ? lds_nt(g.lam + c) : 0.f;
-      mu_[q] = comp[q] ? lds_nt(g.mu + c) : 0.f;
+      la[q] = nla[q]; mu_[q] = nmu[q];
 #pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const long e = (TXX + f) * fs + c;
-        float mv = (comp[q] && KIND != 2) ? lds_nt(KIND == 0 ? W + e : mm + e) : 0.f;
-        if (surf && comp[q] && KIND != 2 && (f == 2 || f == 4 || f == 5))
-          mv = g.msurf[(long)slot * 3 * g.ps + (f == 2 ? 0 : (f == 4 ? 1 : 2)) * g.ps + cc[q]];
-        mo[q][f] = mv;
-        to[q][f] = (comp[q] && KIND != 0) ? lds_nt(tt + e) : 0.f;
+      for (int f = 0; f < 6; ++f) { mo[q][f] = nmo[q][f]; to[q][f] = nto[q][f]; }
+    }
+    if (RK_AHEAD) load_rk(kn); else load_rk(kz);
+    if (!RK_AHEAD) {
+#pragma unroll
+      for (int q = 0; q < RY; ++q) {
+        la[q] = nla[q]; mu_[q] = nmu[q];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) { mo[q][f] = nmo[q][f]; to[q][f] = nto[q][f]; }
       }
     }
-    // prefetch plane k+1
-    const bool more = k < k1;
-    const unsigned kn = kz + sz;
+    // prefetch plane k+1 (the last plane re-reads plane k: every address valid)
     float lead[RY][3];
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const unsigned l = cc[q] + kn + (ZF ? 3u : 1u) * sz;
-      const bool p = more && ok[q];
-      lead[q][0] = p ? pV0[l] : 0.f;
-      lead[q][1] = p ? pV1[l] : 0.f;
-      lead[q][2] = p ? pV2[l] : 0.f;
+      lead[q][0] = ldv(pV0, l, ok[q]);
+      lead[q][1] = ldv(pV1, l, ok[q]);
+      lead[q][2] = ldv(pV2, l, ok[q]);
     }
-    float nx0 = 0.f, nx1 = 0.f, nx2 = 0.f, ny0 = 0.f, ny1 = 0.f, ny2 = 0.f;
-    if (more) {
-      nx0 = ld(pV0, hx, kn); nx1 = ld(pV1, hx, kn); nx2 = ld(pV2, hx, kn);
-      ny0 = ld(pV0, hy, kn); ny1 = ld(pV1, hy, kn); ny2 = ld(pV2, hy, kn);
-    }
+    const float nx0 = ld(pV0, hx, kn), nx1 = ld(pV1, hx, kn), nx2 = ld(pV2, hx, kn);
+    const float ny0 = ld(pV0, hy, kn), ny1 = ld(pV1, hy, kn), ny2 = ld(pV2, hy, kn);
     AuxVals pv[RY];
     if constexpr (PML) {
 #pragma unroll
       for (int q = 0; q < RY; ++q)
-        if (pre[q].face >= 0) aux_load<KIND, AUX_FULL_STR>(pre[q], k, k0, g.setin, pv[q]);
+        if (pre[q].face >= 0) aux_load<FORM, AUX_FULL_STR>(pre[q], k, k0, g.setin, pv[q]);
     }
-    __syncthreads();
+    // plane metric with the plane's other loads (used right after the barrier)
     const float ze = g.zeta_z[k], rzk = g.rz[k];
+    __syncthreads();
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
@@ -772,16 +871,35 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
         h[4] = 2.0f * miu * E13;
         h[5] = 2.0f * miu * E23;
         if constexpr (PML)
-          if (pml_here) pml_str<KIND>(g, slot, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
+          if (pml_here) pml_str<FORM>(g, slot, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
                         sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
+        if (surf) {
+          // Tzz Txz Tyz are imaged at k = nk2: their m and W_1 / W_3 are the
+          // copies k_surface saved before imaging
+#pragma unroll
+          for (int f = 2; f < 6; ++f) {
+            if (f == 3) continue;
+            const int sp = f == 2 ? 0 : (f == 4 ? 1 : 2);
+            mo[q][f] = ms[sp * g.ps + cc[q]];
+          }
+        }
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
           const long e = (TXX + f) * fs + c;
-          if constexpr (KIND == 2) {
-            st_nt(wout + e, to[q][f] + g.b * h[f]);
-          } else {
+          const bool img = surf && (f == 2 || f == 4 || f == 5);
+          const int sp = f == 2 ? 0 : (f == 4 ? 1 : 2);
+          float wc = st[LP[f] * PL + o];  // the stage input's own value
+          if constexpr (FORM == 1 || FORM == 3)
+            if (img) wc = ms[(3 + sp) * g.ps + cc[q]];
+          if constexpr (FORM == 0) {
+            st_nt(wout + e, (img ? mo[q][f] : wc) + g.a * h[f]);
+          } else if constexpr (FORM == 1) {
             st_nt(wout + e, mo[q][f] + g.a * h[f]);
-            st_nt(tt + e, (KIND == 0 ? mo[q][f] : to[q][f]) + g.b * h[f]);
+            st_nt(tt + e, mo[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
+          } else if constexpr (FORM == 2) {
+            st_nt(wout + e, mo[q][f] + g.a * h[f]);
+          } else {
+            st_nt(wout + e, to[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
           }
         }
       }
@@ -797,62 +915,67 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
   }
 }
 
-// One launch per stage and kind: the narrow edge tiles run the narrow
-// body (with the CFS-PML corrections), the wide PML-shell tiles the PML
-// body, the wide interior tiles the lean body.  At most 168 VGPRs (3 waves
-// per SIMD): the inner RK kind of k_vel then keeps a few values in scratch,
-// which measures faster than 2 waves per SIMD.  With 4-row tiles the first
-// and last RK kinds of k_vel fit 96 VGPRs (5 waves per SIMD; 0.4% faster
-// per stage, the inner kind and k_str spill too much there).
-template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
-__global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(BYT * RY == 4 && KIND != 1 ? 5 : 3))) void k_vel(StageArgs g) {
-  constexpr int SW = VelLds<BYT, RY, TX>::SIZE, SN = VelLds<BYT, RY, TXN>::SIZE;
-  __shared__ float sh[SW > SN ? SW : SN];
+// LDS of one block: the velocity group's planes, then the stress group's,
+// each sized for the larger of the wide and narrow tiles.  Each group's region ends with TX spare words (halo writes of lanes
+// without a halo slot).
+template <int BYT, int RY>
+struct StageLds {
+  static constexpr int VW = 2 * VNL * TileGeom<BYT, RY, TX>::PL, VN = 2 * VNL * TileGeom<BYT, RY, TXN>::PL;
+  static constexpr int SW = 2 * SNL * TileGeom<BYT, RY, TX>::PL, SN = 2 * SNL * TileGeom<BYT, RY, TXN>::PL;
+  static constexpr int VOFF = (VW > VN ? VW : VN) + TX, SIZE = VOFF + (SW > SN ? SW : SN) + TX;
+};
+
+// One launch per RK stage: blocks of 2 x BYT waves, the velocity group
+// first.  The narrow edge tiles run the narrow body (with the CFS-PML
+// corrections), the wide PML-shell tiles the PML body, the wide interior
+// tiles the lean body.  At most 128 VGPRs (4 waves per SIMD: two 8-wave
+// blocks per CU).
+template <int XF, int YF, int ZF, int FORM, int BYT, int RY>
+__global__ __launch_bounds__(2 * TX * BYT) __attribute__((amdgpu_waves_per_eu(4))) void k_stage(StageArgs g) {
+  using L = StageLds<BYT, RY>;
+  __shared__ float sh[L::SIZE];
   const ItemPos ip = item_pos(g);
-  const int L = ip.item, slot = g.slot0 + ip.member;
-  const int4 it = g.items[L];
-  if (L < g.p1)
-    vel_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, slot, sh);
-  else if (L < g.p2)
-    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, slot, sh);
-  else
-    vel_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, slot, sh);
-}
-template <int XF, int YF, int ZF, int KIND, int BYT, int RY>
-__global__ __launch_bounds__(TX* BYT) __attribute__((amdgpu_waves_per_eu(3))) void k_str(StageArgs g) {
-  constexpr int SW = StrLds<BYT, RY, TX>::SIZE, SN = StrLds<BYT, RY, TXN>::SIZE;
-  __shared__ float sh[SW > SN ? SW : SN];
-  const ItemPos ip = item_pos(g);
-  const int L = ip.item, slot = g.slot0 + ip.member;
-  const int4 it = g.items[L];
-  if (L < g.p1)
-    str_body<XF, YF, ZF, KIND, BYT, RY, TXN, true>(g, it, slot, sh);
-  else if (L < g.p2)
-    str_body<XF, YF, ZF, KIND, BYT, RY, TX, true>(g, it, slot, sh);
-  else
-    str_body<XF, YF, ZF, KIND, BYT, RY, TX, false>(g, it, slot, sh);
+  const int it_ = ip.item, slot = g.slot0 + ip.member;
+  const int4 it = g.items[it_];
+  // wave index, uniform over the wave: the group choice is a scalar branch
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.y);
+  float* const sv = sh;
+  float* const ss = sh + L::VOFF;
+  if (wave < BYT) {
+    if (it_ < g.p1)
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TXN, true>(g, it, slot, sv, ss, wave);
+    else if (it_ < g.p2)
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, true>(g, it, slot, sv, ss, wave);
+    else
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, false>(g, it, slot, sv, ss, wave);
+  } else {
+    if (it_ < g.p1)
+      str_body<XF, YF, ZF, FORM, BYT, RY, TXN, true>(g, it, slot, ss, sv, wave - BYT);
+    else if (it_ < g.p2)
+      str_body<XF, YF, ZF, FORM, BYT, RY, TX, true>(g, it, slot, ss, sv, wave - BYT);
+    else
+      str_body<XF, YF, ZF, FORM, BYT, RY, TX, false>(g, it, slot, ss, sv, wave - BYT);
+  }
 }
 
 // Stage-kernel tile shapes (the autotuner picks among them; the first is the
-// default): 4-row tiles of 4 waves with 1 row per thread, 8-row tiles of 4
-// waves with 2 rows per thread, 16-row tiles of 8 waves, 8-row tiles of 8
-// waves with 1 row per thread.  At 400x400x200 with PML: 1250, 1293, 1481,
-// 1259 us per stage.
+// default): waves per group x rows per thread = 4-row tiles (4 x 1), 8-row
+// tiles (4 x 2, 8 x 1), 16-row tiles (8 x 2).
 constexpr int NTILECFG = 4;
-constexpr int TILE_BYT[NTILECFG] = {4, 4, 8, 8};  // waves per block
+constexpr int TILE_BYT[NTILECFG] = {4, 4, 8, 8};  // waves per group (a block holds two groups)
 constexpr int TILE_RY[NTILECFG] = {1, 2, 2, 1};   // rows per thread
 
-// The 8 F/B patterns x 3 RK kinds of one stage kernel (0: k_vel, 1: k_str)
-// and tile shape are instantiated in a translation unit of their own
-// (fw_stage_inst.hip, compiled once per kernel and shape, in parallel).
+// The 8 F/B patterns x 4 RK forms of one tile shape are instantiated in
+// translation units of their own (fw_stage_inst.hip, compiled once per shape
+// and pattern half, in parallel).
 typedef void (*StageFn)(StageArgs);
-StageFn stage_fn_v0(int pat, int kind);
-StageFn stage_fn_v1(int pat, int kind);
-StageFn stage_fn_v2(int pat, int kind);
-StageFn stage_fn_v3(int pat, int kind);
-StageFn stage_fn_s0(int pat, int kind);
-StageFn stage_fn_s1(int pat, int kind);
-StageFn stage_fn_s2(int pat, int kind);
-StageFn stage_fn_s3(int pat, int kind);
+StageFn stage_fn_0_0(int pat, int form);
+StageFn stage_fn_0_1(int pat, int form);
+StageFn stage_fn_1_0(int pat, int form);
+StageFn stage_fn_1_1(int pat, int form);
+StageFn stage_fn_2_0(int pat, int form);
+StageFn stage_fn_2_1(int pat, int form);
+StageFn stage_fn_3_0(int pat, int form);
+StageFn stage_fn_3_1(int pat, int form);
 
 }  // namespace fw

@@ -1,39 +1,33 @@
-// Instantiations of one stage kernel (k_vel or k_str, FW_KERNEL 0 / 1) at
-// one tile shape (FW_TILE 0..3) over the 8 F/B patterns x 3 RK kinds.  The
-// build compiles this file once per (kernel, shape) into separate objects so
-// that the 192 stage kernels compile in parallel; each object exports
-// fw::stage_fn_{v,s}<shape>(pattern, kind), the host stub of the kernel.
+// Instantiations of the stage kernel at one tile shape (FW_TILE 0..3) for
+// half of the 8 F/B patterns (FW_PHALF 0: patterns 0-3, 1: 4-7) x the 4 RK
+// forms.  The build compiles this file once per (shape, half) into separate
+// objects so that the 128 stage kernels compile in parallel; each object
+// exports fw::stage_fn_<shape>_<half>(pattern, form), the host stub of the
+// kernel.
 #include "fw_stage.h"
 
-#if !defined(FW_KERNEL) || !defined(FW_TILE)
-#error "compile with -DFW_KERNEL=0|1 -DFW_TILE=0..3"
+#if !defined(FW_TILE) || !defined(FW_PHALF)
+#error "compile with -DFW_TILE=0..3 -DFW_PHALF=0|1"
 #endif
 
-#define FW_CAT2(a, b) a##b
-#define FW_CAT(a, b) FW_CAT2(a, b)
-#if FW_KERNEL == 0
-#define FW_FN FW_CAT(stage_fn_v, FW_TILE)
-#else
-#define FW_FN FW_CAT(stage_fn_s, FW_TILE)
-#endif
+#define FW_CAT4(a, b, c, d) a##b##c##d
+#define FW_NAME(t, h) FW_CAT4(stage_fn_, t, _, h)
 
 namespace fw {
 
-template <int P, int K>
+StageFn FW_NAME(FW_TILE, FW_PHALF)(int pat, int form);
+
+template <int P, int F>
 static StageFn one() {
   constexpr int XF = (P >> 2) & 1, YF = (P >> 1) & 1, ZF = P & 1, B = TILE_BYT[FW_TILE], R = TILE_RY[FW_TILE];
-#if FW_KERNEL == 0
-  return k_vel<XF, YF, ZF, K, B, R>;
-#else
-  return k_str<XF, YF, ZF, K, B, R>;
-#endif
+  return k_stage<XF, YF, ZF, F, B, R>;
 }
-#define FW_ROW(P) {one<P, 0>(), one<P, 1>(), one<P, 2>()}
+#define FW_ROW(P) {one<P, 0>(), one<P, 1>(), one<P, 2>(), one<P, 3>()}
 
-StageFn FW_FN(int pat, int kind) {
-  static const StageFn T[8][3] = {FW_ROW(0), FW_ROW(1), FW_ROW(2), FW_ROW(3),
-                                  FW_ROW(4), FW_ROW(5), FW_ROW(6), FW_ROW(7)};
-  return T[pat & 7][kind];
+StageFn FW_NAME(FW_TILE, FW_PHALF)(int pat, int form) {
+  constexpr int P0 = 4 * FW_PHALF;
+  static const StageFn T[4][4] = {FW_ROW(P0), FW_ROW(P0 + 1), FW_ROW(P0 + 2), FW_ROW(P0 + 3)};
+  return T[(pat - P0) & 3][form & 3];
 }
 
 }  // namespace fw
