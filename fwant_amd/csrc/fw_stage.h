@@ -165,11 +165,14 @@ __device__ __forceinline__ float* aux_base(const FaceArgs& f, int slot) {
   return f.d + (long)slot * 24 * f.cnt;
 }
 
-// Auxiliary values of three variables of one face cell, loaded ahead of
-// use: psi of the stage's input set and, when FULL, the set-A and set-T
-// values the RK form reads.
+// Auxiliary values of three variables of the first face slab holding a cell
+// (see aux_plane), loaded with the plane's other loads: psi of the stage's
+// input set, when FULL the set-A and set-T values the RK form reads, and the
+// face axis' profile values 1/B, A + D/B, D/B.
 struct AuxVals {
   float p[3], a[3], t[3];
+  float b1, ad, dd;
+  int face;  // 0..5, or -1: the cell is in no slab
 };
 __device__ __forceinline__ float sel3(int m, float v0, float v1, float v2) { return m == 0 ? v0 : (m == 1 ? v1 : v2); }
 
@@ -201,12 +204,13 @@ __device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, i
   return aux_base(f, slot)[g.setin * 6 * f.cnt + v * f.cnt + li];
 }
 
-// Prefetch of the auxiliary values of the first x or y face slab holding a
-// cell.  x and y slabs span every plane of the volume, so the face and its
-// index step per plane are fixed for the column; the loads of plane k are
-// issued with the RK operands, before the plane's barrier.  Cells in more
-// than one slab (edges, corners) load the other slabs' values where they are
-// applied.
+// Prefetch of the auxiliary values of the first face slab holding a cell,
+// in the reference's face order x1 x2 y1 y2 z1 z2.  x and y slabs span
+// every plane of the volume, so their face and index step per plane are
+// fixed for the column (AuxPre); a cell in no x or y slab takes the z slab
+// of its plane.  The loads of plane k are issued with the RK operands,
+// before the plane's barrier; cells in more than one slab (edges, corners)
+// load the other slabs' values where they are applied.
 struct AuxPre {
   const float* base;  // set A, first of the three variables, this cell, plane k0
   int face;           // 0..3, or -1: no x or y slab
@@ -235,19 +239,43 @@ __device__ __forceinline__ int face_mask(const StageArgs& g, const AuxPre& a, in
   return a.mask | (in_face(g.face[4], i, j, k) ? 16 : 0) | (in_face(g.face[5], i, j, k) ? 32 : 0);
 }
 template <int FORM, bool FULL>
-__device__ __forceinline__ void aux_load(const AuxPre& a, int k, int k0, int setin, AuxVals& r) {
-  const float* b = a.base + (long)(k - k0) * a.kst;
-  const long c = a.cnt;
+__device__ __forceinline__ void aux_plane(const StageArgs& g, const AuxPre& a, int slot, int i, int j, int k, int k0, int v0,
+                                          bool comp, AuxVals& r) {
+  const float* b = nullptr;
+  long c = 0;
+  int ia = 0;
+  r = AuxVals{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f, 0.f, -1};
+  if (a.face >= 0) {
+    r.face = a.face;
+    b = a.base + (long)(k - k0) * a.kst;
+    c = a.cnt;
+    ia = a.face < 2 ? i : g.nx + j;
+  } else if (comp && in_face(g.face[4], i, j, k)) {
+    r.face = 4;
+    c = g.face[4].cnt;
+    b = aux_base(g.face[4], slot) + (long)v0 * c + face_index(g.face[4], i, j, k);
+    ia = g.nx + g.ny + k;
+  } else if (comp && in_face(g.face[5], i, j, k)) {
+    r.face = 5;
+    c = g.face[5].cnt;
+    b = aux_base(g.face[5], slot) + (long)v0 * c + face_index(g.face[5], i, j, k);
+    ia = g.nx + g.ny + k;
+  }
+  if (r.face >= 0) {
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    r.p[m] = b[setin * 6 * c + m * c];
-    r.a[m] = FULL && FORM != 0 ? b[m * c] : 0.f;
-    r.t[m] = FULL && FORM == 3 ? b[18 * c + m * c] : 0.f;
+    for (int m = 0; m < 3; ++m) {
+      r.p[m] = b[g.setin * 6 * c + m * c];
+      r.a[m] = FULL && FORM != 0 ? b[m * c] : 0.f;
+      r.t[m] = FULL && FORM == 3 ? b[18 * c + m * c] : 0.f;
+    }
+    r.b1 = g.pb1[ia];
+    r.ad = g.pad[ia];
+    r.dd = g.pdd[ia];
   }
 }
-// velocity group of the wide tiles: psi, set A and set T ahead; narrow
-// tiles and the stress group (more live RK operands): psi only
-constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = false;
+// psi, set A and set T ahead in the wide tiles of both groups; narrow tiles:
+// psi only
+constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = true;
 
 // CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
 // reference's face order x1 x2 y1 y2 z1 z2 (mod_abs_npml.F90:1149-2123, the
@@ -264,9 +292,9 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int slot, int i, int
     if (!((mask >> n) & 1)) continue;
     const int ax = n >> 1;
     const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
-    const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
-    const long li = face_index(f, i, j, k);
     const bool pre = n == pf;
+    const float b1 = pre ? pv.b1 : g.pb1[ia], b1_1 = b1 - 1.0f, ad = pre ? pv.ad : g.pad[ia], Dd = pre ? pv.dd : g.pdd[ia];
+    const long li = face_index(f, i, j, k);
     const float p3 = pre ? pv.p[0] : aux_in(f, g, slot, li, 3), p4 = pre ? pv.p[1] : aux_in(f, g, slot, li, 4),
                 p5 = pre ? pv.p[2] : aux_in(f, g, slot, li, 5);
     float D0, D1, D2;  // corrected derivatives feeding hVx, hVy, hVz
@@ -297,7 +325,7 @@ __device__ __forceinline__ void pml_vel(const StageArgs& g, int slot, int i, int
 // CFS-PML corrections of hT (velocity-derivative half), including the
 // free-surface Hooke fix-ups of the x and y faces (mod_abs_npml.F90:1303-1333).
 // h: Txx Tyy Tzz Txy Txz Tyz.  DzV* are the standard 5-tap z derivatives.
-template <int FORM>
+template <int FORM, bool FULL>
 __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int j, int k, float h[6], float lam, float miu,
                                         float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
                                         float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
@@ -311,9 +339,9 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int
     if (!((mask >> n) & 1)) continue;
     const int ax = n >> 1;
     const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
-    const float b1 = g.pb1[ia], b1_1 = b1 - 1.0f, ad = g.pad[ia], Dd = g.pdd[ia];
-    const long li = face_index(f, i, j, k);
     const bool pre = n == pf;
+    const float b1 = pre ? pv.b1 : g.pb1[ia], b1_1 = b1 - 1.0f, ad = pre ? pv.ad : g.pad[ia], Dd = pre ? pv.dd : g.pdd[ia];
+    const long li = face_index(f, i, j, k);
     const float p0 = pre ? pv.p[0] : aux_in(f, g, slot, li, 0), p1 = pre ? pv.p[1] : aux_in(f, g, slot, li, 1),
                 p2 = pre ? pv.p[2] : aux_in(f, g, slot, li, 2);
     if (ax == 0) {
@@ -327,9 +355,9 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int
         h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
         h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
       }
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else if (ax == 1) {
       const float D0 = (DyVx * rxs - Vy * xc) * rzk;
       const float D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
@@ -343,18 +371,18 @@ __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int
         h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
         h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
       }
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
     } else {
       h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
       h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
       h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
       h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
       h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<FORM, AUX_FULL_STR>(f, g, slot, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
+      aux_rk<FORM, FULL>(f, g, slot, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
     }
   }
 }
@@ -615,8 +643,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
     AuxVals pv[RY];
     if constexpr (PML) {
 #pragma unroll
-      for (int q = 0; q < RY; ++q)
-        if (pre[q].face >= 0) aux_load<FORM, FULLV>(pre[q], k, k0, g.setin, pv[q]);
+      for (int q = 0; q < RY; ++q) aux_plane<FORM, FULLV>(g, pre[q], slot, i, j0 + ty + RS * q, k, k0, 3, comp[q], pv[q]);
     }
     // plane metric with the plane's other loads (used right after the barrier)
     const float ze = g.zeta_z[k], rzk = g.rz[k];
@@ -649,7 +676,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         h[2] = rr[q] * (DxTxz * rzk + DyTyz * rzk * rxs + DzTzz + (2.0f * Tzz - Txx - Tyy + Txz * xc) * rzk);
         if constexpr (PML)
           if (pml_here) pml_vel<FORM, FULLV>(g, slot, i, j0 + ty + RS * q, k, h, rr[q], DxTxx, rDxTxy, rDxTxz, DyTyy, rDyTxy, rDyTyz, DzTzz,
-                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
+                        rDzTxz, rDzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pv[q].face, pv[q]);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
           const long e = f * fs + c;
@@ -696,6 +723,7 @@ template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML>
 __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
   using G = TileGeom<BYT, RY, TW>;
   constexpr int RS = BYT * (TX / TW), TY = G::TY, LW = G::LW, PL = G::PL;
+  constexpr bool FULLS = AUX_FULL_STR && TW == TX;
   // LDS plane of each output field (Txx Tyy Tzz Txy Txz Tyz) in the velocity group's planes
   constexpr int LP[6] = {LTXX, LTYY, LTZZ, LTXY, LTXZ, LTYZ};
   const int tx = threadIdx.x % TW, ty = threadIdx.x / TW + (TX / TW) * wy, tid = threadIdx.x + TX * wy;
@@ -825,8 +853,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
     AuxVals pv[RY];
     if constexpr (PML) {
 #pragma unroll
-      for (int q = 0; q < RY; ++q)
-        if (pre[q].face >= 0) aux_load<FORM, AUX_FULL_STR>(pre[q], k, k0, g.setin, pv[q]);
+      for (int q = 0; q < RY; ++q) aux_plane<FORM, FULLS>(g, pre[q], slot, i, j0 + ty + RS * q, k, k0, 0, comp[q], pv[q]);
     }
     // plane metric with the plane's other loads (used right after the barrier)
     const float ze = g.zeta_z[k], rzk = g.rz[k];
@@ -871,8 +898,8 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
         h[4] = 2.0f * miu * E13;
         h[5] = 2.0f * miu * E23;
         if constexpr (PML)
-          if (pml_here) pml_str<FORM>(g, slot, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
-                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pre[q].face, pv[q]);
+          if (pml_here) pml_str<FORM, FULLS>(g, slot, i, j0 + ty + RS * q, k, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, sDzVx, sDzVy,
+                        sDzVz, Vx, Vy, Vz, rzk, rxs, xc, face_mask(g, pre[q], i, j0 + ty + RS * q, k), pv[q].face, pv[q]);
         if (surf) {
           // Tzz Txz Tyz are imaged at k = nk2: their m and W_1 / W_3 are the
           // copies k_surface saved before imaging
