@@ -706,6 +706,12 @@ int fw_set_force(fw_handle h, int n, int ntw, int stf_id, const int* indx, const
                  const float* stf_time, const float* stf_freq, const float* Fx, const float* Fy, const float* Fz) {
   if (!h || n < 0 || (n > 0 && (ntw <= 0 || !indx || !shift || !t0 || !stf_time || !stf_freq || !Fx || !Fy || !Fz)))
     return fail(h, FW_ERR_ARG, "fw_set_force: bad argument");
+  if (n == 0) {  // no forces: the arrays may be NULL and ntw is ignored
+    h->nforce = 0; h->fntw = 0; h->fstf = stf_id;
+    h->findx.clear(); h->fshift.clear(); h->ft0.clear(); h->ftime.clear(); h->ffreq.clear(); h->F.clear();
+    h->ready = false;
+    return FW_OK;
+  }
   h->nforce = n; h->fntw = ntw; h->fstf = stf_id;
   h->findx.assign(indx, indx + 3 * n);
   h->fshift.assign(shift, shift + 3 * n);
@@ -724,6 +730,12 @@ int fw_set_moment(fw_handle h, int n, int ntw, int stf_id, const int* indx, cons
                   const float* stf_time, const float* stf_freq, const float* M6) {
   if (!h || n < 0 || (n > 0 && (ntw <= 0 || !indx || !shift || !t0 || !stf_time || !stf_freq || !M6)))
     return fail(h, FW_ERR_ARG, "fw_set_moment: bad argument");
+  if (n == 0) {  // no moment sources: the arrays may be NULL and ntw is ignored
+    h->nmom = 0; h->mntw = 0; h->mstf = stf_id;
+    h->mindx.clear(); h->mshift.clear(); h->mt0.clear(); h->mtime.clear(); h->mfreq.clear(); h->M.clear();
+    h->ready = false;
+    return FW_OK;
+  }
   h->nmom = n; h->mntw = ntw; h->mstf = stf_id;
   h->mindx.assign(indx, indx + 3 * n);
   h->mshift.assign(shift, shift + 3 * n);

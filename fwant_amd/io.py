@@ -113,6 +113,12 @@ class RunSetup:
     snapshots: list = field(default_factory=list)
     output_root: str = "."
     snap_tcnt: list = field(default_factory=list)
+    # restart keys (mod_io.F90:101-103): rest files under checkpoint_root
+    # every checkpoint_tinv steps; run_from_checkpoint > 0 starts from
+    # rest_t<that step>
+    checkpoint_root: str = "."
+    checkpoint_tinv: int = 0
+    run_from_checkpoint: int = 0
 
 
 def _sources(path, kind):
@@ -212,12 +218,36 @@ def load_run(conf_path):
         m.rec_tinv = int(nc.tinv)
     metric = {k: v.astype(np.float32) for k, v in met.items()}
     snaps, tcnt = _snapshot_defs(conf, ni, nj, nk, 3 + nk)
-    return RunSetup(m, metric, snaps, root("OUTPUT_ROOT"), tcnt)
+    ck_root = root("CHECKPOINT_ROOT") if conf.has("CHECKPOINT_ROOT") else root("OUTPUT_ROOT")
+    ck_tinv = conf.get("checkpoint_tinv", 2, int) if conf.has("checkpoint_tinv") else 0
+    ck_from = conf.get("run_from_checkpoint", 2, int) if conf.has("run_from_checkpoint") else 0
+    if ck_tinv < 0 or ck_from < 0:
+        raise ValueError("checkpoint_tinv and run_from_checkpoint must be >= 0")
+    return RunSetup(m, metric, snaps, root("OUTPUT_ROOT"), tcnt, ck_root, ck_tinv, ck_from)
 
 
-def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None):
+def rest_path(root, ntime):
+    """rest_tNNNNN_mpi000000.nc of step ntime (get_fnm_rest, mod_io.F90)."""
+    return os.path.join(root, f"rest_t{ntime:05d}_{SUFFIX}.nc")
+
+
+def run_plan(nt, ntime0, tinv):
+    """Steps of a reference-style run from ntime0 and the steps after which
+    it writes rest files.  The driver tests ntime > nt only at the top of
+    each 8-step block (seis3d_wave_8op_p3.F90:92-95), so it runs
+    8*ceil((nt - ntime0 + 1)/8) steps; io_rest_export writes rest_t<ntime>
+    after every step whose new ntime is a multiple of checkpoint_tinv
+    (mod_io.F90:364-374; tinv 0: never)."""
+    steps = 8 * ((nt - ntime0 + 1 + 7) // 8) if ntime0 <= nt else 0
+    marks = [t for t in range(ntime0 + 1, ntime0 + steps + 1) if tinv > 0 and t % tinv == 0]
+    return steps, marks
+
+
+def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None, checkpoint_tinv=0,
+                 run_from_checkpoint=0):
     """Write SeisFD3D.conf and the input files of `model` under `root`
-    (directories input/{grid,media,source,station} and output/)."""
+    (directories input/{grid,media,source,station}, output/ and, with the
+    restart keys, checkpoint/)."""
     dirs = {k: os.path.join(root, "input", k) for k in ("grid", "media", "source", "station")}
     for d in list(dirs.values()) + [os.path.join(root, "output")]:
         os.makedirs(d, exist_ok=True)
@@ -282,6 +312,10 @@ def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None
         "STATION_ROOT = input/station", "OUTPUT_ROOT = output",
         f"number_of_snap = {len(snapshots)}",
     ]
+    if checkpoint_tinv or run_from_checkpoint:
+        os.makedirs(os.path.join(root, "checkpoint"), exist_ok=True)
+        lines += ["CHECKPOINT_ROOT = checkpoint", f"checkpoint_tinv = {int(checkpoint_tinv)}",
+                  f"run_from_checkpoint = {int(run_from_checkpoint)}"]
     for q, s in enumerate(snapshots, 1):
         var = ("V" if any(f.startswith("V") for f in s["fields"]) else "") + \
               ("T" if any(f.startswith("T") for f in s["fields"]) else "")
@@ -301,9 +335,10 @@ def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None
 SEISMO_VARS = ("Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
 
 
-def write_seismo(root, records, stept, tinv):
-    """records [nrecord][num_pt][9]: record m (1-based) is step m*tinv at
-    time m*tinv*stept (io_seismo_put)."""
+def write_seismo(root, records, stept, tinv, ntime0=0):
+    """records [nrecord][num_pt][9]: record m (1-based) is step ntime0 +
+    m*tinv at time (ntime0 + m*tinv)*stept (io_seismo_put; ntime0 > 0 for a
+    run restarted from a rest file)."""
     records = np.asarray(records, np.float32)
     nrec, npt = records.shape[:2]
     path = _fnm(root, "seismo")
@@ -314,7 +349,7 @@ def write_seismo(root, records, stept, tinv):
         t = nc.createVariable("time", "f4", ("time",))
         vs = [nc.createVariable(v, "f4", ("time", "num_pt")) for v in SEISMO_VARS]
         if nrec:
-            t[:nrec] = (np.arange(1, nrec + 1) * tinv * np.float32(stept)).astype(np.float32)
+            t[:nrec] = ((ntime0 + np.arange(1, nrec + 1) * tinv) * np.float32(stept)).astype(np.float32)
             for f, v in enumerate(vs):
                 v[:nrec] = records[:, :, f]
     return path
@@ -368,7 +403,7 @@ def write_rest(root, ntime, fields, state=None, cycle_origin=None):
     ghosted wavefields (dims I, J, K).  With `state` (Solver.checkpoint())
     the file also carries the complete state for an exact restart, which
     the reference's reader ignores."""
-    path = os.path.join(root, f"rest_t{ntime:05d}_{SUFFIX}.nc")
+    path = rest_path(root, ntime)
     nx, ny, nz = np.asarray(fields["Vx"]).shape
     with netcdf_file(path, "w", version=2) as nc:
         nc.title = "Restart file generated by seis3d_wave"

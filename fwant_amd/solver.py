@@ -157,10 +157,14 @@ class Solver:
         check(self.L.fw_set_batch_width(self.h, int(width)), self.h, "fw_set_batch_width")
 
     def autotune(self, trial_steps=4):
-        """Pick the fastest tile shape and batch width; returns (tile, width, ms per step)."""
+        """Pick the fastest tile shape and batch width; returns (tile, width, ms per step).
+        fw_autotune restores the zero initial state and empty records, so the
+        step counter starts again at 0: autotune before run(), or expect the
+        next run() to start from rest at ntime 0."""
         t, w, ms = C.c_int(), C.c_int(), C.c_float()
         check(self.L.fw_autotune(self.h, int(trial_steps), C.byref(t), C.byref(w), C.byref(ms)), self.h,
               "fw_autotune")
+        self.ntime = 0
         return t.value, w.value, ms.value
 
     def sync(self):

@@ -85,12 +85,14 @@ int fw_set_media(fw_handle h, const float *lambda, const float *mu, const float 
 
 /* Replaces src_import for force sources (code/srcF/mod_src.F90:168-185):
  *   indx[3*n] (1-based local), shift[3*n], t0[n], stf_time[ntw], stf_freq[ntw],
- *   Fx/Fy/Fz[ntw*n] (window index fastest). */
+ *   Fx/Fy/Fz[ntw*n] (window index fastest).  n = 0: no force sources; the
+ *   arrays may be NULL and ntw is ignored. */
 int fw_set_force(fw_handle h, int n, int ntw, int stf_id, const int *indx, const float *shift,
                  const float *t0, const float *stf_time, const float *stf_freq, const float *Fx,
                  const float *Fy, const float *Fz);
 /* Replaces src_import for moment sources (mod_src.F90:150-167):
- *   M6[6*ntw*n] = Mxx, Myy, Mzz, Mxy, Mxz, Myz blocks, each [ntw*n]. */
+ *   M6[6*ntw*n] = Mxx, Myy, Mzz, Mxy, Mxz, Myz blocks, each [ntw*n].  n = 0: no
+ *   moment sources; the arrays may be NULL and ntw is ignored. */
 int fw_set_moment(fw_handle h, int n, int ntw, int stf_id, const int *indx, const float *shift,
                   const float *t0, const float *stf_time, const float *stf_freq, const float *M6);
 /* Replaces io_pt_import + io_seismo_init (code/srcF/mod_io.F90:160-176, 430-459):

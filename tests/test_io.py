@@ -113,3 +113,24 @@ def test_output_schemas(tmp_path):
         np.testing.assert_allclose(nc.variables["time"][:], [3 * 3 * np.float32(0.05), 4 * 3 * np.float32(0.05)])
         assert list(nc.subs) == [3, 2, 4] and list(nc.subc) == [4, 3, 2]
         assert float(nc.stept) == pytest.approx(0.15)
+
+
+def test_restart_keys_and_step_plan(tmp_path):
+    """run_from_checkpoint / checkpoint_tinv / CHECKPOINT_ROOT (mod_io.F90:101-103)
+    and the steps of a reference-style run: blocks of 8 until ntime > nt,
+    rest files after every step whose ntime is a multiple of checkpoint_tinv."""
+    from oracle.pyoracle import build_metric
+    m = models.small_test(ni=20, nj=18, nk=16, nt=31, pml=4)
+    met = {k: v.astype(np.float32) for k, v in build_metric(m).items()}
+    conf = io.write_inputs(m, str(tmp_path), metric=met, checkpoint_tinv=8, run_from_checkpoint=16)
+    s = io.load_run(conf)
+    assert (s.checkpoint_tinv, s.run_from_checkpoint) == (8, 16)
+    assert s.checkpoint_root == os.path.join(str(tmp_path), "checkpoint")
+    assert io.run_plan(31, 16, 8) == (16, [24, 32])
+    assert io.run_plan(15, 0, 8) == (16, [8, 16])
+    assert io.run_plan(47, 0, 0) == (48, [])
+    assert io.run_plan(1000, 0, 0)[0] == 1008
+    assert io.run_plan(20, 17, 5) == (8, [20, 25])
+    assert io.run_plan(10, 11, 5) == (0, [])
+    plain = io.load_run(io.write_inputs(m, str(tmp_path / "p"), metric=met))
+    assert (plain.checkpoint_tinv, plain.run_from_checkpoint) == (0, 0)
