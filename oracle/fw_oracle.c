@@ -20,6 +20,11 @@
  * seismogram variables in code/srcF/mod_io.F90:436-444).
  *
  * Parity status of each part (what pins it) is stated at each function.
+ * The coupled step is pinned by Stokes' point-force and moment-tensor
+ * solutions (tests/test_oracle_analytic.py).  Parity unpinned: the
+ * amplitude of the free-surface response to a surface force at the percent
+ * level (a Lamb's-problem comparison did not converge); its one-stage
+ * injection and the surface's Rayleigh speed and reciprocity are pinned.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -793,8 +798,12 @@ static void abs_correct(orc_solver *s, int xf, int yf, int zf) {
 
 /* ------------------------------------------------------------------------
  * a7: sources, code/srcF/mod_src.F90:207-414 (SrcSmooth + SrcSurface build).
- * Pinned by tests/test_oracle_src.py (Gaussian weights sum to one, total
- * injected impulse equals F*stf, surface traction = F/S*stf).
+ * Pinned by tests/test_oracle_src.py (Gaussian weights sum to one) and
+ * tests/test_oracle_src_injection.py (one stage's terms through src_terms:
+ * total injected force F*stf against exact spherical cell volumes, the cut
+ * and renormalised spread below a free surface, surface traction times the
+ * exact cell area = F*stf and its rate, moment sum = -M*rate), and by
+ * Stokes' solutions (tests/test_oracle_analytic.py).
  * ---------------------------------------------------------------------- */
 static real cell_volume(orc_solver *s, int i, int j, int k) {
   return (s->x[i + 1] - s->x[i - 1]) * (s->y[j + 1] - s->y[j - 1]) * (s->z[k + 1] - s->z[k - 1]) / C_(8.0) *
@@ -887,6 +896,22 @@ static void src_stress(orc_solver *s, int ntime, real tinc) {
           }
     }
   }
+}
+
+/* The source terms of one stage alone (test entry): h after src_surface,
+ * src_stress and src_force on a zero h (hout: 9*ncell, field-major), and
+ * the surface traction / traction-rate planes TxSrc TySrc TzSrc VxSrc VySrc
+ * VzSrc (surf: 6*nx*ny) that free_timg and the traction condition read. */
+void ORC(src_terms)(orc_solver *s, int ntime, real tinc, real *hout, real *surf) {
+  long n2 = (long)s->nx * s->ny;
+  for (int f = 0; f < NFIELD; f++) memset(s->h[f], 0, sizeof(real) * s->ncell);
+  real *sp[6] = {s->TxSrc, s->TySrc, s->TzSrc, s->VxSrc, s->VySrc, s->VzSrc};
+  for (int q = 0; q < 6; q++) memset(sp[q], 0, sizeof(real) * n2);
+  src_surface(s, ntime, tinc);
+  src_stress(s, ntime, tinc);
+  src_force(s, ntime, tinc);
+  for (int f = 0; f < NFIELD; f++) memcpy(hout + (long)f * s->ncell, s->h[f], sizeof(real) * s->ncell);
+  for (int q = 0; q < 6; q++) memcpy(surf + q * n2, sp[q], sizeof(real) * n2);
 }
 
 /* ------------------------------------------------------------------------

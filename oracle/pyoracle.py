@@ -70,6 +70,7 @@ def _load(prec):
         "f32_to_fp16": (C.c_uint16, [C.c_float]),
         "fp16_to_f32": (C.c_float, [C.c_uint16]),
         "set_mutation": (None, [C.c_int]),
+        "src_terms": (None, [vp, C.c_int, R, rp, rp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, p + name)
@@ -239,6 +240,17 @@ class OracleSolver:
         out = np.zeros(9 * n, self.lib.dtype)
         self.lib.rhs(self.h, xf, yf, zf, self.lib.ptr(out))
         return out.reshape((9, m.nz, m.ny, m.nx)).transpose(0, 3, 2, 1)
+
+    def src_terms(self, ntime, tinc):
+        """Source terms of one stage at time (ntime + tinc) dt: h [9][nx][ny][nz]
+        and the surface planes TxSrc TySrc TzSrc VxSrc VySrc VzSrc [6][nx][ny]."""
+        m = self.model
+        n = m.nx * m.ny * m.nz
+        h = np.zeros(9 * n, self.lib.dtype)
+        surf = np.zeros(6 * m.nx * m.ny, self.lib.dtype)
+        self.lib.src_terms(self.h, int(ntime), float(tinc), self.lib.ptr(h), self.lib.ptr(surf))
+        return (h.reshape((9, m.nz, m.ny, m.nx)).transpose(0, 3, 2, 1),
+                surf.reshape((6, m.ny, m.nx)).transpose(0, 2, 1))
 
     def pml_profiles(self):
         m = self.model
