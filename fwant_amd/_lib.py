@@ -19,7 +19,7 @@ FIELDS = ("Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
 SYMBOLS = (
     "fw_create", "fw_destroy", "fw_last_error", "fw_dims", "fw_set_grid", "fw_build_metric",
     "fw_set_media", "fw_set_force", "fw_set_moment", "fw_set_receivers", "fw_add_snapshot",
-    "fw_init", "fw_get_pml_profiles", "fw_step", "fw_run", "fw_reference_steps", "fw_sync",
+    "fw_init", "fw_get_pml_profiles", "fw_step", "fw_stage", "fw_run", "fw_reference_steps", "fw_sync",
     "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_add_snapshot_packed",
     "fw_get_snapshot_packed", "fw_snapshot_packed_device", "fwk_sgt_decode", "fw_set_batch",
     "fw_select_member", "fw_set_batch_width", "fw_use_graphs", "fw_autotune", "fw_set_cycle_origin",
@@ -73,6 +73,7 @@ def _sig(lib):
         "fw_init": (C.c_int, [H]),
         "fw_get_pml_profiles": (C.c_int, [H, fp, fp, fp]),
         "fw_step": (C.c_int, [H, C.c_int]),
+        "fw_stage": (C.c_int, [H, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "fw_run": (C.c_int, [H, C.c_int, C.c_int]),
         "fw_reference_steps": (C.c_int, [C.c_int]),
         "fw_sync": (C.c_int, [H]),

@@ -43,14 +43,17 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 
 
 def build_fortran(verbose: bool = True) -> str:
-    """Fortran driver over the ISO_C_BINDING module (fortran/fwant_c.f90),
-    linked against the in-tree libfwant.so (rpath $ORIGIN/..)."""
+    """Fortran driver over the ISO_C_BINDING module (fortran/fwant_c.f90)
+    and the drop-in loop modules (fortran/fwant_shim.f90), linked against the
+    in-tree libfwant.so (rpath $ORIGIN/..)."""
     bdir = os.path.join(FORTRAN, "_build")
     os.makedirs(bdir, exist_ok=True)
     obj = os.path.join(bdir, "fwant_c.o")
+    shim = os.path.join(bdir, "fwant_shim.o")
     steps = [
         [FLANG, "-O2", "-module-dir", bdir, "-c", os.path.join(FORTRAN, "fwant_c.f90"), "-o", obj],
-        [FLANG, "-O2", "-module-dir", bdir, os.path.join(FORTRAN, "fw_drive.f90"), obj, "-L" + HERE, "-lfwant",
+        [FLANG, "-O2", "-module-dir", bdir, "-c", os.path.join(FORTRAN, "fwant_shim.f90"), "-o", shim],
+        [FLANG, "-O2", "-module-dir", bdir, os.path.join(FORTRAN, "fw_drive.f90"), obj, shim, "-L" + HERE, "-lfwant",
          "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"],
     ]
     for cmd in steps:

@@ -439,6 +439,7 @@ struct fw_solver_s {
   int* d_fluidcol = nullptr;  // per (i, j) column: 1 if any cell has mu <= SEIS_ZERO
   float* d_stage = nullptr;   // dense staging copy of one volume (host transfers)
   int tilecfg = 0;            // stage-kernel tile shape index
+  int stage_next = 0, stage_ntime = 0, stage_nact = 0;  // fw_stage: next stage, its step, running members
   int4* d_items = nullptr;    // stage work items: PML-shell items, then interior items
   int n_narrow = 0, n_lean = 0, n_pml = 0;
   int xg = 1;  // stage-kernel items per XCD run
@@ -1010,6 +1011,7 @@ static int zero_state(fw_handle h) {
   HIPCHK(hipStreamSynchronize(h->stream));
   h->nrec = 0;
   for (auto& s : h->snaps) s.nrec = 0;
+  h->stage_next = 0;
   return FW_OK;
 }
 
@@ -1246,11 +1248,16 @@ static int build_items(fw_handle h) {
 
 extern "C" {
 
-static int launch_stage(fw_handle h, int ntime, int st, int nact) {
-  const int* bp = BLOCK_PAT[(((ntime - h->cycle_origin) % 8) + 8) % 8];
-  const int flip = st & 1;
-  const int xf = flip ? !bp[0] : bp[0], yf = flip ? !bp[1] : bp[1], zf = flip ? !bp[2] : bp[2];
-  const int pat = (xf << 2) | (yf << 1) | zf;
+// One RK stage st (0..3) of step ntime for the nact running members, with
+// the F/B pattern of the 8-step cycle, or pattern `pat` ((xf << 2) | (yf << 1)
+// | zf) when it is >= 0.
+static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) {
+  if (pat < 0) {
+    const int* bp = BLOCK_PAT[(((ntime - h->cycle_origin) % 8) + 8) % 8];
+    const int flip = st & 1;
+    const int xf = flip ? !bp[0] : bp[0], yf = flip ? !bp[1] : bp[1], zf = flip ? !bp[2] : bp[2];
+    pat = (xf << 2) | (yf << 1) | zf;
+  }
   // register rotation: stage in/out sets
   static const int IN[4] = {0, 1, 2, 1}, OUT[4] = {1, 2, 1, 0};
   float* win = h->set[IN[st]];
@@ -1455,6 +1462,35 @@ static int set_clock(fw_handle h, int ntime) {
 
 int fw_step(fw_handle h, int ntime) { return fw_run(h, ntime, 1); }
 
+// One RK stage at a time, in the order a Fortran driver calls the
+// reference's routines (see fwant.h).  Stage 0 starts step ntime (device
+// clock, output accounting), stage 3 ends it (receivers, snapshots, clock
+// tick); the stages of a step must come in order 0, 1, 2, 3.
+int fw_stage(fw_handle h, int ntime, int stage, int xf, int yf, int zf) {
+  if (!h) return FW_ERR_ARG;
+  if (!h->ready) return fail(h, FW_ERR_STATE, "fw_stage: call fw_init first");
+  if (ntime < 0 || stage < 0 || stage > 3 || (xf | yf | zf) & ~1)
+    return fail(h, FW_ERR_ARG, "fw_stage: stage must be 0..3, ntime >= 0 and the F/B flags 0 or 1");
+  if (stage != h->stage_next || (stage > 0 && ntime != h->stage_ntime))
+    return fail(h, FW_ERR_STATE, "fw_stage: the stages of a step must run in order 0, 1, 2, 3");
+  int rc;
+  if (stage == 0) {
+    h->stage_nact = running(h, ntime);
+    h->stage_ntime = ntime;
+    if (h->stage_nact > 0) {
+      if ((rc = set_clock(h, ntime))) return rc;
+      if ((rc = account_outputs(h, ntime, h->stage_nact))) return rc;
+    }
+  }
+  if (h->stage_nact > 0) {
+    if ((rc = launch_stage(h, ntime, stage, h->stage_nact, (xf << 2) | (yf << 1) | zf))) return rc;
+    if (stage == 3) enqueue_outputs(h, ntime, h->stage_nact, false);
+    HIPCHK(hipGetLastError());
+  }
+  h->stage_next = (stage + 1) & 3;
+  return FW_OK;
+}
+
 int fw_enable_timing(fw_handle h, int on) {
   if (!h) return FW_ERR_ARG;
   h->timing = on != 0;
@@ -1465,6 +1501,7 @@ int fw_run(fw_handle h, int ntime0, int nsteps) {
   if (!h) return FW_ERR_ARG;
   if (!h->ready) return fail(h, FW_ERR_STATE, "fw_run: call fw_init first");
   if (nsteps < 0 || ntime0 < 0) return fail(h, FW_ERR_ARG, "fw_run: ntime0 and nsteps must be >= 0");
+  if (h->stage_next != 0) return fail(h, FW_ERR_STATE, "fw_run: a step begun with fw_stage is not finished");
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (h->timing) {
     const size_t need = 2 * (size_t)nsteps * 4;

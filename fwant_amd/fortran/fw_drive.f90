@@ -5,6 +5,14 @@
 !
 !   fw_drive CASE OUT   run the case file CASE, write the receiver records
 !                       (float32 [nsteps][nrec][9]) to OUT
+!   fw_drive CASE OUT stages
+!                       the same, with the time loop written as the
+!                       reference's loop body (seis3d_wave_8op_p3.F90:
+!                       97-157): macdrp_syn, abs_syn and per RK stage
+!                       set_cur_time, src_surface, macdrp_Lx?_Ly?_Lz?,
+!                       abs_Lx?_Ly?_Lz?, src_stress, src_force,
+!                       macdrp_RK_*, abs_RK_*, calling the drop-in modules of
+!                       fwant_shim.f90 (one fw_stage per RK stage)
 !   fw_drive            probe: print the step count for nt = 1000 and the
 !                       status of fw_create (a message, not a stop, when
 !                       the machine has no GPU)
@@ -20,7 +28,17 @@
 program fw_drive
   use iso_c_binding
   use fwant_c
+  use macdrp_mod
+  use abs_mod
+  use src_mod
   implicit none
+  ! operator pattern (x << 2 | y << 1 | z, 1 = forward) of the first RK stage
+  ! of each step of the 8-step block: BBB FFB FFF BBF BFB FBB FBF BFF
+  ! (seis3d_wave_8op_p3.F90:97-591); stages 2 and 4 flip all three
+  integer, parameter :: block_pat(0:7) = (/ 0, 6, 7, 1, 2, 4, 5, 3 /)
+  integer :: ntime
+  logical :: use_shim
+  character(len=16) :: mode
   type(fw_config) :: cfg
   type(c_ptr) :: h
   integer(c_int) :: hdr(15), rc, nrecords
@@ -43,6 +61,9 @@ program fw_drive
   end if
   call get_command_argument(1, case_path)
   call get_command_argument(2, out_path)
+  mode = ''
+  if (command_argument_count() >= 3) call get_command_argument(3, mode)
+  use_shim = trim(mode) == 'stages'
 
   open(newunit=u, file=trim(case_path), access='stream', form='unformatted', status='old')
   read(u) hdr
@@ -76,7 +97,14 @@ program fw_drive
                             h, 'fw_set_moment')
   call fw_check(fw_set_receivers(h, nrec, tinv, rindx, nsteps), h, 'fw_set_receivers')
   call fw_check(fw_init(h), h, 'fw_init')
-  call fw_check(fw_run(h, 0, nsteps), h, 'fw_run')
+  if (use_shim) then
+    call fwant_shim_attach(h)
+    do ntime = 0, nsteps - 1
+      call rk_step(ntime, block_pat(mod(ntime, 8)), cfg%stept)
+    end do
+  else
+    call fw_check(fw_run(h, 0, nsteps), h, 'fw_run')
+  end if
   allocate(rec(9 * nrec * nsteps))
   call fw_check(fw_get_receivers(h, rec, nrecords), h, 'fw_get_receivers')
   call fw_check(fw_destroy(h), c_null_ptr, 'fw_destroy')
@@ -85,4 +113,77 @@ program fw_drive
   write(u) rec(1:9 * nrec * nrecords)
   close(u)
   write(*, '(a, i0, a, i0, a)') 'fw_drive: ', nrecords, ' records of ', nrec, ' receivers'
+
+contains
+
+  ! the loop body of seis3d_wave_8op_p3.F90:97-157 for a step whose first
+  ! RK stage uses pattern p
+  subroutine rk_step(ntime, p, stept)
+    integer, intent(in) :: ntime, p
+    real(c_float), intent(in) :: stept
+    call macdrp_syn
+    call abs_syn
+    ! the 1th stage
+    call set_cur_time(ntime, 0.0_c_float)
+    call src_surface(ntime, 0.0_c_float, stept)
+    call sweep(p)
+    call src_stress(hTxx, hTyy, hTzz, hTxy, hTxz, hTyz, ntime, 0.0_c_float, stept)
+    call src_force(hVx, hVy, hVz, ntime, 0.0_c_float, stept)
+    call macdrp_RK_beg(firRKa(1), firRKb(1))
+    call abs_RK_beg(firRKa(1), firRKb(1))
+    ! the 2th stage
+    call set_cur_time(ntime, firRKa(1))
+    call src_surface(ntime, firRKa(1), stept)
+    call sweep(7 - p)
+    call src_stress(hTxx, hTyy, hTzz, hTxy, hTxz, hTyz, ntime, firRKa(1), stept)
+    call src_force(hVx, hVy, hVz, ntime, firRKa(1), stept)
+    call macdrp_RK_inn(firRKa(2), firRKb(2))
+    call abs_RK_inn(firRKa(2), firRKb(2))
+    ! the 3th stage
+    call set_cur_time(ntime, firRKa(2))
+    call src_surface(ntime, firRKa(2), stept)
+    call sweep(p)
+    call src_stress(hTxx, hTyy, hTzz, hTxy, hTxz, hTyz, ntime, firRKa(2), stept)
+    call src_force(hVx, hVy, hVz, ntime, firRKa(2), stept)
+    call macdrp_RK_inn(firRKa(3), firRKb(3))
+    call abs_RK_inn(firRKa(3), firRKb(3))
+    ! the 4th stage
+    call set_cur_time(ntime, firRKa(3))
+    call src_surface(ntime, firRKa(3), stept)
+    call sweep(7 - p)
+    call src_stress(hTxx, hTyy, hTzz, hTxy, hTxz, hTyz, ntime, firRKa(3), stept)
+    call src_force(hVx, hVy, hVz, ntime, firRKa(3), stept)
+    call macdrp_RK_fin(firRKb(4))
+    call abs_RK_fin(firRKb(4))
+  end subroutine
+
+  ! macdrp_Lx?_Ly?_Lz? followed by abs_Lx?_Ly?_Lz? of pattern p
+  subroutine sweep(p)
+    integer, intent(in) :: p
+    select case (p)
+    case (0)
+      call macdrp_LxB_LyB_LzB; call abs_LxB_LyB_LzB
+    case (1)
+      call macdrp_LxB_LyB_LzF; call abs_LxB_LyB_LzF
+    case (2)
+      call macdrp_LxB_LyF_LzB; call abs_LxB_LyF_LzB
+    case (3)
+      call macdrp_LxB_LyF_LzF; call abs_LxB_LyF_LzF
+    case (4)
+      call macdrp_LxF_LyB_LzB; call abs_LxF_LyB_LzB
+    case (5)
+      call macdrp_LxF_LyB_LzF; call abs_LxF_LyB_LzF
+    case (6)
+      call macdrp_LxF_LyF_LzB; call abs_LxF_LyF_LzB
+    case default
+      call macdrp_LxF_LyF_LzF; call abs_LxF_LyF_LzF
+    end select
+  end subroutine
+
+  ! para_mod's current time (mod_para.F90:96-101): the library derives each
+  ! stage's source time from ntime and the stage itself
+  subroutine set_cur_time(ntime, incr)
+    integer, intent(in) :: ntime
+    real(c_float), intent(in) :: incr
+  end subroutine
 end program fw_drive

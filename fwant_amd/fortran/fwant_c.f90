@@ -11,7 +11,7 @@ module fwant_c
   public :: fw_config, fw_check, fw_error_message
   public :: fw_create, fw_destroy, fw_last_error, fw_dims, fw_set_grid, fw_build_metric, fw_set_media
   public :: fw_set_force, fw_set_moment, fw_set_receivers, fw_add_snapshot, fw_init, fw_get_pml_profiles
-  public :: fw_step, fw_run, fw_reference_steps, fw_sync, fw_get_receivers, fw_get_snapshot
+  public :: fw_step, fw_stage, fw_run, fw_reference_steps, fw_sync, fw_get_receivers, fw_get_snapshot
   public :: fw_get_wavefield, fw_set_wavefield, fw_enable_timing, fw_get_timing
 
   integer(c_int), parameter, public :: FW_OK = 0
@@ -108,6 +108,11 @@ module fwant_c
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       integer(c_int), value :: ntime
+    end function
+    integer(c_int) function fw_stage(h, ntime, stage, xf, yf, zf) bind(C, name="fw_stage")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: ntime, stage, xf, yf, zf
     end function
     integer(c_int) function fw_run(h, ntime0, nsteps) bind(C, name="fw_run")
       import :: c_int, c_ptr

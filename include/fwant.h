@@ -118,6 +118,21 @@ int fw_get_pml_profiles(fw_handle h, float *A, float *B, float *D);
  * operator pattern of block (ntime mod 8) (seis3d_wave_8op_p3.F90:82-591),
  * then receiver and snapshot capture for step ntime+1.  Asynchronous. */
 int fw_step(fw_handle h, int ntime);
+/* One RK stage (stage 0..3) of step ntime with the operator pattern the
+ * caller picks (xf, yf, zf: 1 = forward, 0 = backward), for drivers that keep
+ * the reference's loop body (seis3d_wave_8op_p3.F90:97-157).  One call
+ * replaces the stage's src_surface (mod_src.F90:373-414), macdrp_Lx?_Ly?_Lz?
+ * (mod_macdrp.F90:790-2181, with free_timg), abs_Lx?_Ly?_Lz?
+ * (mod_abs_npml.F90:1149-8955), src_stress / src_force (mod_src.F90:207-345)
+ * and macdrp_RK_beg / _inn / _fin with abs_RK_* (mod_macdrp.F90:291-393,
+ * mod_abs_npml.F90:738-1105): stage 0 is RK_beg at tinc 0, stages 1 and 2
+ * RK_inn at tinc a1 / a2, stage 3 RK_fin at tinc a3, with the classical RK4
+ * coefficients (firRKa, firRKb).  Stage 3 also captures receivers and
+ * snapshots for step ntime+1 (io_seismo_put, io_wave_export).  The stages
+ * of a step must come in order 0..3 (FW_ERR_STATE otherwise); fw_run over
+ * the same steps with the 8-step cycle's patterns gives identical results.
+ * Asynchronous. */
+int fw_stage(fw_handle h, int ntime, int stage, int xf, int yf, int zf);
 /* nsteps consecutive steps starting at ntime0. Asynchronous. */
 int fw_run(fw_handle h, int ntime0, int nsteps);
 /* Steps the reference driver runs for a given nt: 8*ceil((nt+1)/8)

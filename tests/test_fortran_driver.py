@@ -70,3 +70,27 @@ def test_fortran_driver_matches_python_and_oracle(gpu_lib, tmp_path):
     e = np.linalg.norm(rec - ref) / np.linalg.norm(ref)
     print(f"Fortran driver vs FP32 oracle: rel-L2 {e:.2e}")
     assert e <= 1e-5
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_reference_loop_body_through_shim_equals_fw_run(gpu_lib, tmp_path):
+    """The time loop written as the reference's loop body
+    (seis3d_wave_8op_p3.F90:97-157: macdrp_syn, abs_syn, and per RK stage
+    src_surface, macdrp_Lx?_Ly?_Lz?, abs_Lx?_Ly?_Lz?, src_stress, src_force,
+    macdrp_RK_*, abs_RK_*) against the drop-in modules of fwant_shim.f90
+    gives bit for bit the records of fw_run over the same steps."""
+    n = 24
+    m = models.small_test(ni=36, nj=32, nk=26, nt=n, seed=33)
+    case = tmp_path / "case.bin"
+    write_case(m, n, case)
+    outs = []
+    for mode in ([], ["stages"]):
+        out = tmp_path / f"rec{len(mode)}.bin"
+        print(f"fw_drive {' '.join(mode) or 'fw_run'} on small_test 36x32x26, {n} steps", flush=True)
+        r = subprocess.run([DRIVER, str(case), str(out)] + mode, capture_output=True, text=True, timeout=120)
+        print(r.stdout, r.stderr)
+        assert r.returncode == 0
+        outs.append(np.fromfile(out, np.float32).reshape(n, len(m.rec_indx), 9))
+    np.testing.assert_array_equal(outs[1], outs[0])
+    assert np.abs(outs[0]).max() > 0

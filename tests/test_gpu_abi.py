@@ -24,3 +24,29 @@ def test_no_sources_with_null_arrays(gpu_lib):
         assert L.fw_set_force(h, 1, 1, 200, None, None, None, None, None, None, None, None) != 0
     finally:
         L.fw_destroy(h)
+
+
+def test_fw_stage_order_and_equivalence(gpu_lib):
+    """fw_stage: stages of a step in order 0..3 (FW_ERR_STATE otherwise, and
+    fw_run refuses to start inside a step); with the 8-step cycle's
+    patterns it reproduces fw_run bit for bit."""
+    import numpy as np
+    from fwant_amd import models
+    from fwant_amd.solver import Solver
+    m = models.small_test(ni=24, nj=22, nk=20, nt=16, seed=35)
+    a = Solver(m)
+    a.run(16)
+    ra = a.receivers()
+    b = Solver(m, metric=a.metric)
+    L = gpu_lib
+    assert L.fw_stage(b.h, 0, 1, 0, 0, 0) != 0  # stage 1 before stage 0
+    pat = [0, 6, 7, 1, 2, 4, 5, 3]  # first-stage patterns of the 8-step block
+    for t in range(16):
+        for st in range(4):
+            p = pat[t % 8] if st % 2 == 0 else 7 - pat[t % 8]
+            assert L.fw_stage(b.h, t, st, (p >> 2) & 1, (p >> 1) & 1, p & 1) == 0
+            if t == 3 and st == 1:
+                assert L.fw_run(b.h, 3, 1) != 0  # inside a step
+    np.testing.assert_array_equal(b.receivers(), ra)
+    a.close()
+    b.close()
