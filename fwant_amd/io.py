@@ -17,12 +17,18 @@ and the per-rank NetCDF-classic files (single rank: suffix mpi000000).
   write_snapshots(...)       sgt_NNN_ / vel_NNN_mpi000000_nNNNNN.nc files of
                              tcnt records each (mod_nfseis.F90:131-203,
                              mod_io.F90:505-645, 825-836)
+  grid3d_create / grid3d_put a 3-D grid file (kernel_NNN_ files) created
+                             once and written block by block in place
+                             (mod_nfseis.F90:208-255, 450-462)
 
-NetCDF-classic reading and writing goes through scipy.io.netcdf_file.
+NetCDF-classic reading and writing goes through scipy.io.netcdf_file,
+except the in-place block writes of grid3d_put (numpy.memmap at the
+variable's offset, read from the file's header).
 Arrays keep the reference's Fortran order: a variable of Fortran shape
 (I, J, K) is stored with the NetCDF dimensions (K, J, I).
 """
 import os
+import struct
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -417,6 +423,123 @@ def write_rest(root, ntime, fields, state=None, cycle_origin=None):
             nc.cycle_origin = np.int32(0 if cycle_origin is None else cycle_origin)
         nc.ntime = np.int32(ntime)
     return path
+
+
+# ---------------------------------------------------------------------------
+# 3-D grid files written block by block (kernel_NNN_ files)
+# ---------------------------------------------------------------------------
+_NC_TYPES = {1: ">i1", 2: "S1", 3: ">i2", 4: ">i4", 5: ">f4", 6: ">f8"}
+
+
+def _pad4(b):
+    return b + b"\0" * (-len(b) % 4)
+
+
+def _nc_name(s):
+    e = s.encode()
+    return struct.pack(">I", len(e)) + _pad4(e)
+
+
+def classic_layout(path):
+    """{name: (begin, shape, big-endian dtype)} of the fixed-size variables
+    of a classic (CDF-1) or 64-bit-offset (CDF-2) NetCDF file, parsed from
+    its header (the header grammar of the NetCDF classic format
+    specification); record variables are left out."""
+    with open(path, "rb") as f:
+        buf = f.read(1 << 20)
+    if buf[:3] != b"CDF" or buf[3] not in (1, 2):
+        raise ValueError(f"{path}: not a classic or 64-bit-offset NetCDF file")
+    off64 = buf[3] == 2
+    p = 8
+
+    def u32():
+        nonlocal p
+        v = struct.unpack_from(">I", buf, p)[0]
+        p += 4
+        return v
+
+    def name():
+        nonlocal p
+        n = u32()
+        s = buf[p:p + n].decode()
+        p += (n + 3) // 4 * 4
+        return s
+
+    def atts():
+        nonlocal p
+        u32()
+        for _ in range(u32()):
+            name()
+            t = u32()
+            ne = u32()
+            p += (ne * np.dtype(_NC_TYPES[t]).itemsize + 3) // 4 * 4
+
+    u32()
+    dims = []
+    for _ in range(u32()):
+        dims.append((name(), u32()))
+    atts()
+    out = {}
+    u32()
+    for _ in range(u32()):
+        nm = name()
+        ids = [u32() for _ in range(u32())]
+        atts()
+        t = u32()
+        u32()  # vsize
+        if off64:
+            begin = struct.unpack_from(">Q", buf, p)[0]
+            p += 8
+        else:
+            begin = u32()
+        shape = tuple(dims[i][1] for i in ids)
+        if ids and dims[ids[0]][1] == 0:
+            continue  # record variable
+        out[nm] = (begin, shape, np.dtype(_NC_TYPES[t]))
+    return out
+
+
+def grid3d_create(path, subc, names, title):
+    """A 64-bit-offset NetCDF file with dimensions I, J, K = subc and float
+    variables `names` (K, J, I), all zero (nfseis_grid3d_def/defvar,
+    code/srcF/mod_nfseis.F90:208-255; the reference creates its files with
+    no fill, so points it never writes are undefined there).  The header is
+    written here and the data region is extended with zeros, so a large
+    file costs no memory."""
+    ni, nj, nk = (int(v) for v in subc)
+    vsize = 4 * ni * nj * nk
+    hdr = b"CDF\x02" + struct.pack(">I", 0)
+    hdr += struct.pack(">II", 0x0A, 3) + b"".join(_nc_name(d) + struct.pack(">I", n)
+                                                  for d, n in zip("IJK", (ni, nj, nk)))
+    t = title.encode()
+    hdr += struct.pack(">II", 0x0C, 1) + _nc_name("title") + struct.pack(">II", 2, len(t)) + _pad4(t)
+    var_len = sum(len(_nc_name(v)) + 4 + 12 + 8 + 8 + 8 for v in names)
+    begin0 = len(hdr) + 8 + var_len
+    hdr += struct.pack(">II", 0x0B, len(names))
+    for q, v in enumerate(names):
+        hdr += _nc_name(v) + struct.pack(">IIII", 3, 2, 1, 0) + struct.pack(">II", 0, 0)
+        hdr += struct.pack(">II", 5, vsize) + struct.pack(">Q", begin0 + q * vsize)
+    assert len(hdr) == begin0
+    with open(path, "wb") as f:
+        f.write(hdr)
+        f.truncate(begin0 + len(names) * vsize)
+    return path
+
+
+def grid3d_put(path, name, data, subs, subc, subt):
+    """Write `data` (the points of the sub-grid subs/subc/subt, 1-based,
+    i fastest) into variable `name` of a grid file in place (nfseis_put,
+    code/srcF/mod_nfseis.F90:450-462)."""
+    begin, shape, dt = classic_layout(path)[name]
+    sl = []
+    for s, c, t, n in zip(subs[::-1], subc[::-1], subt[::-1], shape):
+        if s < 1 or c < 1 or t < 1 or s + (c - 1) * t > n:
+            raise ValueError(f"{path}: {name} sub-grid {subs} {subc} {subt} outside {shape[::-1]}")
+        sl.append(slice(s - 1, s + (c - 1) * t, t))
+    mm = np.memmap(path, dtype=dt, mode="r+", offset=begin, shape=shape)
+    mm[tuple(sl)] = np.asarray(data, np.float32).reshape(subc[2], subc[1], subc[0])
+    mm.flush()
+    del mm
 
 
 def read_rest(path):
