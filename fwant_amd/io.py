@@ -249,38 +249,50 @@ def run_plan(nt, ntime0, tinv):
     return steps, marks
 
 
-def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None, checkpoint_tinv=0,
-                 run_from_checkpoint=0):
-    """Write SeisFD3D.conf and the input files of `model` under `root`
-    (directories input/{grid,media,source,station}, output/ and, with the
-    restart keys, checkpoint/)."""
-    dirs = {k: os.path.join(root, "input", k) for k in ("grid", "media", "source", "station")}
-    for d in list(dirs.values()) + [os.path.join(root, "output")]:
-        os.makedirs(d, exist_ok=True)
-    m = model
-    nx, ny, nz = m.nx, m.ny, m.nz
-    with netcdf_file(_fnm(dirs["grid"], "coord"), "w") as nc:
-        for n, v in zip("IJK", (nx, ny, nz)):
+STF_NAMES = {100: "bell_int", 110: "triangle_int", 120: "gauss_int", 200: "ricker", 210: "B(shift)",
+             220: "gauss", 230: "bell", 240: "step", 250: "delta"}  # stf_id2name, mod_src.F90:651-677
+
+
+def write_coord(path, x, y, z):
+    """coord_mpi000000.nc: x (colatitude, rad), y (longitude, rad), z
+    (radius, m) on the ghosted axes (seis3d_grid.F90:135-177)."""
+    with netcdf_file(path, "w") as nc:
+        for n, v in zip("IJK", (len(x), len(y), len(z))):
             nc.createDimension(n, v)
-        for n, d, v in (("x", "I", m.x), ("y", "J", m.y), ("z", "K", m.z)):
+        for n, d, v in (("x", "I", x), ("y", "J", y), ("z", "K", z)):
             nc.createVariable(n, "f4", (d,))[:] = np.asarray(v, np.float32)
-    if metric is None:
-        from .solver import build_metric
-        metric = build_metric(m)
-    with netcdf_file(_fnm(dirs["grid"], "metric"), "w") as nc:
+    return path
+
+
+def write_metric(path, metric, nx, ny, nz):
+    with netcdf_file(path, "w") as nc:
         for n, v in zip("IJK", (nx, ny, nz)):
             nc.createDimension(n, v)
         for n, d in (("xsin", "I"), ("xcot", "I"), ("xi_x", "I"), ("eta_y", "J"), ("zeta_z", "K")):
             nc.createVariable(n, "f4", (d,))[:] = np.asarray(metric[n], np.float32)
-    with netcdf_file(_fnm(dirs["media"], "media"), "w", version=2) as nc:
+    return path
+
+
+def write_media(path, lam, mu, rho):
+    """media_mpi000000.nc: lambda, mu, rho on the ghosted grid (media_create)."""
+    nx, ny, nz = np.asarray(lam).shape
+    with netcdf_file(path, "w", version=2) as nc:
         for n, v in zip("IJK", (nx, ny, nz)):
             nc.createDimension(n, v)
-        for n, a in (("lambda", m.lam), ("mu", m.mu), ("rho", m.rho)):
+        for n, a in (("lambda", lam), ("mu", mu), ("rho", rho)):
             nc.createVariable(n, "f4", ("K", "J", "I"))[:] = np.asarray(a, np.float32).T
-    with netcdf_file(_fnm(dirs["source"], "source"), "w") as nc:
+    return path
+
+
+def write_source(path, force, moment, force_axis=None, moment_axis=None):
+    """source_mpi000000.nc (srcnode_skel and the puts of seis3d_source.F90:
+    211-306, 428-493): per kind the point and time-window dimensions, the
+    component tables (window fastest), 1-based ghosted indices, located
+    coordinates (axis), shifts, start times, STF factors, id and name."""
+    with netcdf_file(path, "w") as nc:
         nc.createDimension("geo_dimension", 3)
-        for kind, s, names in (("force", m.force, ("Fx", "Fy", "Fz")),
-                               ("moment", m.moment, ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"))):
+        for kind, s, names, axis in (("force", force, ("Fx", "Fy", "Fz"), force_axis),
+                                     ("moment", moment, ("Mxx", "Myy", "Mzz", "Mxy", "Mxz", "Myz"), moment_axis)):
             setattr(nc, f"number_of_{kind}", np.int32(s.n))
             if s.n <= 0:
                 continue
@@ -290,20 +302,57 @@ def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None
                 nc.createVariable(v, "f4", (f"{kind}_point", f"{kind}_time_window"))[:] = \
                     np.asarray(s.comps[q], np.float32).reshape(s.n, s.ntw)
             nc.createVariable(f"{kind}_indx", "i4", (f"{kind}_point", "geo_dimension"))[:] = s.indx
+            if axis is not None:
+                nc.createVariable(f"{kind}_axis", "f4", (f"{kind}_point", "geo_dimension"))[:] = \
+                    np.asarray(axis, np.float32).reshape(s.n, 3)
             nc.createVariable(f"{kind}_shift", "f4", (f"{kind}_point", "geo_dimension"))[:] = s.shift
             nc.createVariable(f"{kind}_start_time", "f4", (f"{kind}_point",))[:] = s.t0
             nc.createVariable(f"{kind}_stf_time", "f4", (f"{kind}_time_window",))[:] = s.stf_time
             nc.createVariable(f"{kind}_stf_freq", "f4", (f"{kind}_time_window",))[:] = s.stf_freq
             setattr(nc, f"{kind}_stf_id", np.int32(s.stf_id))
-    npt = len(m.rec_indx)
-    with netcdf_file(_fnm(dirs["station"], "station"), "w") as nc:
+            setattr(nc, f"{kind}_stf_type", STF_NAMES.get(int(s.stf_id), "unknown"))
+    return path
+
+
+def write_station(path, indx, tinv, gindx=None, coord=None, grid=None, ids=None):
+    """station_mpi000000.nc (station_skel, seis3d_station.F90:107-150):
+    1-based ghosted indices, interior indices, requested and grid
+    coordinates and (number, line) ids per receiver; tinv attribute."""
+    indx = np.asarray(indx, np.int32).reshape(-1, 3)
+    npt = len(indx)
+    with netcdf_file(path, "w") as nc:
         nc.createDimension("num_pt", None)
         nc.createDimension("geo_dim", 3)
         nc.createDimension("twoelem", 2)
-        v = nc.createVariable("indx", "i4", ("num_pt", "geo_dim"))
-        if npt:
-            v[:npt] = np.asarray(m.rec_indx, np.int32)
-        nc.tinv = np.int32(m.rec_tinv)
+        vs = {"indx": ("i4", indx), "gindx": ("i4", gindx), "coord": ("f4", coord), "grid": ("f4", grid),
+              "id": ("i4", ids)}
+        for name, (t, a) in vs.items():
+            if name != "indx" and a is None:
+                continue
+            v = nc.createVariable(name, t, ("num_pt", "twoelem" if name == "id" else "geo_dim"))
+            if npt:
+                v[:npt] = np.asarray(a, np.float32 if t == "f4" else np.int32)
+        nc.tinv = np.int32(tinv)
+    return path
+
+
+def write_inputs(model, root, metric=None, snapshots=(), snap_tcnt=None, nt=None, checkpoint_tinv=0,
+                 run_from_checkpoint=0):
+    """Write SeisFD3D.conf and the input files of `model` under `root`
+    (directories input/{grid,media,source,station}, output/ and, with the
+    restart keys, checkpoint/)."""
+    dirs = {k: os.path.join(root, "input", k) for k in ("grid", "media", "source", "station")}
+    for d in list(dirs.values()) + [os.path.join(root, "output")]:
+        os.makedirs(d, exist_ok=True)
+    m = model
+    write_coord(_fnm(dirs["grid"], "coord"), m.x, m.y, m.z)
+    if metric is None:
+        from .solver import build_metric
+        metric = build_metric(m)
+    write_metric(_fnm(dirs["grid"], "metric"), metric, m.nx, m.ny, m.nz)
+    write_media(_fnm(dirs["media"], "media"), m.lam, m.mu, m.rho)
+    write_source(_fnm(dirs["source"], "source"), m.force, m.moment)
+    write_station(_fnm(dirs["station"], "station"), m.rec_indx, m.rec_tinv)
     nt = m.nt if nt is None else nt
     lines = [
         "# SeisFD3D.conf written by fwant_amd.io.write_inputs",
