@@ -1291,6 +1291,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) 
   g.fs = h->fs; g.sy = h->sy; g.sz = h->sz; g.ps = h->ps;
   g.ni1 = h->ni1; g.ni2 = h->ni2; g.nj1 = h->nj1; g.nj2 = h->nj2; g.nk1 = h->nk1; g.nk2 = h->nk2;
   g.freenode = h->freenode; g.a = a; g.b = b; g.r = r;
+  g.c = st == 1 ? 1.0f - r - RKB[2] / RKA[2] : 0.f;
   g.nx = h->nx; g.ny = h->ny; g.fluidcol = h->d_fluidcol;
   g.n1 = h->nx + h->ny + h->nz;
   g.pb1 = h->d_pml; g.pad = h->d_pml + g.n1; g.pdd = h->d_pml + 2 * g.n1;

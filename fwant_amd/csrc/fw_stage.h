@@ -18,12 +18,13 @@
 // b_s h_s; W_{n+1} = t + b_3 h_3).  Since b_0 = a_0 / 3 and b_2 = a_2 / 3
 // (a = 1/2, 1/2, 1; b = 1/6, 1/3, 1/3, 1/6), the accumulated b_0 h_0 and
 // b_2 h_2 equal (W_1 - m) / 3 and (W_3 - m) / 3, and the accumulator is
-// written at stage 1 only:
+// written at stage 1 only, with the -m / 3 of stage 3 folded in, so that
+// the last stage needs no m:
 //   form 0: W_1 = m + a_0 h_0                         (m = W_n = stage input)
-//   form 1: W_2 = m + a_1 h_1,  t = m + r (W_1 - m) + b_1 h_1
+//   form 1: W_2 = m + a_1 h_1,  t = c m + r W_1 + b_1 h_1   (c = 1 - 2r)
 //   form 2: W_3 = m + a_2 h_2
-//   form 3: W_{n+1} = t + r (W_3 - m) + b_3 h_3        (r = 1/3)
-// which moves 129 floats per cell and step instead of 156 (the same form
+//   form 3: W_{n+1} = t + r W_3 + b_3 h_3               (r = 1/3)
+// which moves 120 floats per cell and step instead of 156 (the same form
 // runs on the CFS-PML auxiliaries).  Source terms added after a stage
 // (k_src) add a_s dh to W_{s+1} only at forms 0 and 2, so they enter t
 // through the same differences.
@@ -113,6 +114,7 @@ struct StageArgs {
   const int4* __restrict__ items;  // work items (i0 | ncol << 16, j0 | nrow << 16, k0, k1): a tile over a chunk of planes
   int nitems, p1, p2;  // item parts: narrow [0, p1), wide PML [p1, p2), wide interior
   float a, b, r;       // a_s dt, b_s dt, r = b_{s-1} / a_{s-1}
+  float c;             // form 1: the share of m kept in the accumulator, 1 - r - b_2 / a_2
   // CFS-PML: per-axis profiles 1/B, A + D/B, D/B (x: nx, y: ny, z: nz entries)
   const float* __restrict__ pb1;
   const float* __restrict__ pad;
@@ -188,15 +190,16 @@ __device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, in
   if constexpr (FORM == 0) {
     d[g.setout * S + q] = psi + g.a * dpsi;
   } else {
-    const float A = FULL && pre ? pa : d[q];
     if constexpr (FORM == 1) {
+      const float A = FULL && pre ? pa : d[q];
       d[g.setout * S + q] = A + g.a * dpsi;
-      d[3 * S + q] = A + g.r * (psi - A) + g.b * dpsi;
+      d[3 * S + q] = g.c * A + g.r * psi + g.b * dpsi;
     } else if constexpr (FORM == 2) {
+      const float A = FULL && pre ? pa : d[q];
       d[g.setout * S + q] = A + g.a * dpsi;
     } else {
       const float T = FULL && pre ? pt : d[3 * S + q];
-      d[g.setout * S + q] = T + g.r * (psi - A) + g.b * dpsi;
+      d[g.setout * S + q] = T + g.r * psi + g.b * dpsi;
     }
   }
 }
@@ -265,7 +268,7 @@ __device__ __forceinline__ void aux_plane(const StageArgs& g, const AuxPre& a, i
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       r.p[m] = b[g.setin * 6 * c + m * c];
-      r.a[m] = FULL && FORM != 0 ? b[m * c] : 0.f;
+      r.a[m] = FULL && (FORM == 1 || FORM == 2) ? b[m * c] : 0.f;
       r.t[m] = FULL && FORM == 3 ? b[18 * c + m * c] : 0.f;
     }
     r.b1 = g.pb1[ia];
@@ -566,7 +569,7 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
         const long e = f * fs + c;
-        nmo[q][f] = FORM != 0 ? lds_nt(mm + e) : 0.f;
+        nmo[q][f] = FORM == 1 || FORM == 2 ? lds_nt(mm + e) : 0.f;
         nto[q][f] = FORM == 3 ? lds_nt(tt + e) : 0.f;
       }
     }
@@ -685,11 +688,11 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
             st_nt(wout + e, wc + g.a * h[f]);
           } else if constexpr (FORM == 1) {
             st_nt(wout + e, mo[q][f] + g.a * h[f]);
-            st_nt(tt + e, mo[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
+            st_nt(tt + e, g.c * mo[q][f] + g.r * wc + g.b * h[f]);
           } else if constexpr (FORM == 2) {
             st_nt(wout + e, mo[q][f] + g.a * h[f]);
           } else {
-            st_nt(wout + e, to[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
+            st_nt(wout + e, to[q][f] + g.r * wc + g.b * h[f]);
           }
         }
       }
@@ -788,7 +791,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
 #pragma unroll
       for (int f = 0; f < 6; ++f) {
         const long e = (TXX + f) * fs + c;
-        nmo[q][f] = FORM != 0 ? lds_nt(mm + e) : 0.f;
+        nmo[q][f] = FORM == 1 || FORM == 2 ? lds_nt(mm + e) : 0.f;
         nto[q][f] = FORM == 3 ? lds_nt(tt + e) : 0.f;
       }
     }
@@ -922,11 +925,11 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
             st_nt(wout + e, (img ? mo[q][f] : wc) + g.a * h[f]);
           } else if constexpr (FORM == 1) {
             st_nt(wout + e, mo[q][f] + g.a * h[f]);
-            st_nt(tt + e, mo[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
+            st_nt(tt + e, g.c * mo[q][f] + g.r * wc + g.b * h[f]);
           } else if constexpr (FORM == 2) {
             st_nt(wout + e, mo[q][f] + g.a * h[f]);
           } else {
-            st_nt(wout + e, to[q][f] + g.r * (wc - mo[q][f]) + g.b * h[f]);
+            st_nt(wout + e, to[q][f] + g.r * wc + g.b * h[f]);
           }
         }
       }
