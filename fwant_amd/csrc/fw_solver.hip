@@ -385,11 +385,14 @@ __global__ void k_metric(const double* __restrict__ g, int n, int lo, int hi, fl
   inv[i] = (float)(1.0 / ((b + f) * 0.5));
 }
 
+// grid-stride fill: a dispatch holds fewer than 2^32 work-items, so a batch's
+// register sets (tens of billions of floats) take several passes per thread
 template <typename T>
 __global__ void k_fill(T* p, long n, T v) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) p[q] = v;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) p[q] = v;
 }
+unsigned fill_blocks(long n) { return (unsigned)std::min<long>((n + 255) / 256, 1L << 20); }
 
 // 8-block operator cycle, seis3d_wave_8op_p3.F90:82-591 (1 = forward)
 const int BLOCK_PAT[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
@@ -999,13 +1002,13 @@ static int build_sources(fw_handle h) {
 
 static int zero_state(fw_handle h) {
   const long n = (long)h->nbatch * h->bstride + h->off;
-  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, h->stream, h->d_sets, n, 0.f);
+  hipLaunchKernelGGL(k_fill<float>, dim3(fill_blocks(n)), dim3(256), 0, h->stream, h->d_sets, n, 0.f);
   const long ns = 6L * h->ps * h->nbatch;
-  hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, h->stream, h->d_msurf, ns, 0.f);
+  hipLaunchKernelGGL(k_fill<float>, dim3(fill_blocks(ns)), dim3(256), 0, h->stream, h->d_msurf, ns, 0.f);
   for (auto& f : h->face)
     if (f.on) {
       const long m = 4L * 6 * f.cnt * h->nbatch;
-      hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, h->stream, f.d, m, 0.f);
+      hipLaunchKernelGGL(k_fill<float>, dim3(fill_blocks(m)), dim3(256), 0, h->stream, f.d, m, 0.f);
     }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -1316,7 +1319,9 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) 
   g.p2 = h->n_narrow + h->n_pml;
   g.xg = h->xg;
   g.bstride = h->bstride;
-  const int W = h->width > 0 ? h->width : nact;
+  // a dispatch holds fewer than 2^32 work-items: at most maxm members per launch
+  const long maxm = std::max(1L, 0xffffffffL / ((long)std::max(1, g.nitems) * sblk.x * sblk.y));
+  const int W = (int)std::min<long>(h->width > 0 ? h->width : nact, maxm);
   for (int s0 = 0; s0 < nact && g.nitems > 0; s0 += W) {  // members s0 .. s0+W-1 in one launch
     g.slot0 = s0;
     g.nm = std::min(W, nact - s0);

@@ -1,0 +1,153 @@
+"""The batched and reduced-precision configurations at their own sizes.
+
+Batched virtual sources (256 x 256 x 128 gradient model, h = 0.5 km): 32
+vertical-force sources at seeded surface stations, record lengths uniform
+in 1500-4000 steps scaled by 1/100 to bound the run (15-40 steps); one
+launch per RK stage for the running members, step graphs, tile shape and
+batch width from the autotuner.  Every member equals a solve of its own
+source alone, bit for bit (receiver records and the final Vz and Txz
+volumes for all 32; the BF16 SGT codes for four of them).
+
+Reduced-precision SGT: the 6-component stress SGT every 10 steps on the
+2x decimated grid (128 x 128 x 64) stored as BF16 and FP16 codes with
+per-16^3-block FP32 scales, for 8 members of 120-150 steps (12-15
+records): the codes and scales equal the oracle codec applied to the FP32
+SGT of the same run, bit for bit (two members), and the traveltime kernels
+of 16 station pairs computed on the GPU over the whole decimated grid from
+the BF16 and FP16 SGT stay within 1% relative L2 of the FP64 oracle
+pipeline on the FP32 SGT, compared per pair on 2048 seeded points that
+both SGTs reach (stations within 12 cells of the centre).
+"""
+import numpy as np
+import pytest
+
+from fwant_amd import models
+
+pytestmark = pytest.mark.gpu
+
+N3 = (256, 256, 128)
+
+
+def model3(nt):
+    return models.gradient_400x400x200(nt=nt, ni=N3[0], nj=N3[1], nk=N3[2], h=500.0, seed=4)
+
+
+def stations(n, seed):
+    """n distinct surface stations (0-based ghosted i, j) 24 cells or more inside the grid."""
+    rng = np.random.default_rng(seed)
+    w = N3[0] - 48
+    q = rng.choice(w * (N3[1] - 48), size=n, replace=False)
+    return [(int(3 + 24 + v % w), int(3 + 24 + v // w)) for v in q]
+
+
+def test_config3_batch_of_32_equals_single_solves(gpu_lib):
+    from fwant_amd import pipeline
+    from fwant_amd.solver import Solver
+    st = stations(32, 3)
+    nsteps = [int(v) for v in np.random.default_rng(4).integers(15, 41, size=32)]
+    base = model3(max(nsteps))
+    m, b = models.virtual_sources(base, st, nsteps, f0=1.0)
+    snap = pipeline.sgt_snapshot(m, 5, 2, "bf16", max_records=9)
+    print(f"batch of 32 on {N3}, steps {min(nsteps)}..{max(nsteps)}", flush=True)
+    s = Solver(m, batch=b, snapshots=[snap], max_records=max(nsteps))
+    tile, width, ms = s.autotune(2)
+    print(f"autotuned: tile {tile}, width {width}, {ms:.2f} ms per step", flush=True)
+    s.use_graphs(True)
+    s.run(max(nsteps))
+    s.sync()
+    for q in range(32):
+        s.select(q)
+        one_m, _ = models.virtual_sources(base, [st[q]], [nsteps[q]], f0=1.0)
+        one = Solver(one_m, snapshots=[snap], max_records=max(nsteps))
+        one.run(nsteps[q])
+        rb = s.receivers()
+        assert rb.shape[0] == nsteps[q]
+        np.testing.assert_array_equal(rb, one.receivers())
+        for f in ("Vz", "Txz"):
+            np.testing.assert_array_equal(s.wavefield(f), one.wavefield(f))
+        if q % 8 == 0:
+            cb, sb = s.snapshot_packed(0)
+            c1, s1 = one.snapshot_packed(0)
+            np.testing.assert_array_equal(cb, c1)
+            np.testing.assert_array_equal(sb, s1)
+        one.close()
+    print("32 members identical to their own solves", flush=True)
+    s.close()
+
+
+def test_config4_codes_and_kernel_error(gpu_lib):
+    from scipy import signal
+    from fwant_amd import kernel, pipeline
+    from fwant_amd._lib import DeviceArray
+    from fwant_amd.solver import Solver
+    from oracle.pyoracle import sgt_encode
+    from test_gpu_sgt import oracle_pipeline, rel
+    # 8 stations within 12 cells of the centre, so that every pair's SGTs overlap
+    rng = np.random.default_rng(5)
+    c = 3 + N3[0] // 2
+    st = [(int(c + u), int(c + v)) for u, v in rng.choice(np.arange(-12, 13), size=(8, 2))]
+    assert len(set(st)) == 8
+    nsteps = [int(v) for v in np.random.default_rng(6).integers(120, 151, size=8)]
+    base = model3(max(nsteps))
+    m, b = models.virtual_sources(base, st, nsteps, f0=1.0)
+    m.rec_indx = np.zeros((0, 3), np.int32)
+    snaps = [pipeline.sgt_snapshot(m, 10, 2, f, max_records=16) for f in ("fp32", "bf16", "fp16")]
+    subc = snaps[0]["subc"]
+    assert subc == (128, 128, 64)
+    print(f"8 members on {N3}, steps {min(nsteps)}..{max(nsteps)}, SGT on {subc}", flush=True)
+    s = Solver(m, batch=b, snapshots=snaps, max_records=1)
+    s.use_graphs(True)
+    s.run(max(nsteps))
+    s.sync()
+    npts = int(np.prod(subc))
+    fp32, dec = {}, {}
+    for q in range(8):
+        s.select(q)
+        fp32[q] = s.snapshot(0)  # (nrec, 6, npts)
+        if q < 2:
+            for k, fmt in ((1, "bf16"), (2, "fp16")):
+                codes, scales = s.snapshot_packed(k)
+                oc, osc = sgt_encode(fmt, fp32[q], subc)
+                np.testing.assert_array_equal(scales, osc)
+                np.testing.assert_array_equal(codes, oc)
+            print(f"member {q}: BF16 and FP16 codes equal the oracle codec", flush=True)
+        for k, fmt in ((1, "bf16"), (2, "fp16")):
+            pc, ps, nrec, _, _ = s.snapshot_packed_device(k)
+            y = DeviceArray((nrec, 6, npts), np.float32)
+            kernel.sgt_decode(fmt, pc, ps, subc, nrec * 6, y)
+            dec[q, fmt] = y
+    s.close()
+    sl = tuple(slice(a - 1, a - 1 + c * t, t) for a, c, t in zip((4, 4, 4), subc, (2, 2, 2)))
+    lam = np.ascontiguousarray(base.lam[sl].ravel(order="F"), np.float32)
+    mu = np.ascontiguousarray(base.mu[sl].ravel(order="F"), np.float32)
+    dl, dm = DeviceArray.from_host(lam), DeviceArray.from_host(mu)
+    energy = {q: np.einsum("rcp,rcp->p", fp32[q].astype(np.float64), fp32[q].astype(np.float64)) for q in fp32}
+    prng = np.random.default_rng(7)
+    dt = 10 * base.stept
+    b_, a_ = signal.butter(2, 0.4)
+    rng = np.random.default_rng(8)
+    pairs = [tuple(int(v) for v in rng.choice(8, size=2, replace=False)) for _ in range(16)]
+    worst = {"bf16": 0.0, "fp16": 0.0}
+    for n, (qa, qb) in enumerate(pairs):
+        knt = min(fp32[qa].shape[0], fp32[qb].shape[0])
+        t = dt * np.arange(1, knt + 1)
+        V = np.sin(2 * np.pi * 0.3 * t + n).astype(np.float32)
+        U = np.cos(2 * np.pi * 0.3 * t + n).astype(np.float32)
+        nt1, nt2 = 2, knt - 1
+        # points that both SGTs reach
+        both = np.nonzero((energy[qa] > 1e-6 * energy[qa].max()) & (energy[qb] > 1e-6 * energy[qb].max()))[0]
+        assert len(both) >= 512
+        pts = np.sort(prng.choice(both, size=min(2048, len(both)), replace=False))
+        T = lambda x: np.ascontiguousarray(x[:knt][:, :, pts].transpose(1, 0, 2))  # noqa: E731
+        ref = oracle_pipeline(64, T(fp32[qa]), T(fp32[qb]), lam[pts], mu[pts], dt, V, U, b_, a_, nt1, nt2, 1e16)
+        assert min(np.linalg.norm(r) for r in ref) > 0
+        for fmt in ("bf16", "fp16"):
+            RA = kernel.snapshot_series(dec[qa, fmt].ptr, knt, 6, npts)
+            SB = kernel.snapshot_series(dec[qb, fmt].ptr, knt, 6, npts)
+            got = [o.to_host()[pts] for o in kernel.pair_kernel(None, None, RA, SB, dl, dm, dt, V, U, nt1, nt2,
+                                                                   1e16, 2, filt=("P2", b_, a_))]
+            e = max(rel(g, r) for g, r in zip(got, ref))
+            worst[fmt] = max(worst[fmt], e)
+    print(f"16 pairs: worst kernel rel-L2 vs FP64 oracle: BF16 {worst['bf16']:.2e}, FP16 {worst['fp16']:.2e}",
+          flush=True)
+    assert worst["bf16"] < 1e-2 and worst["fp16"] < 1e-2
