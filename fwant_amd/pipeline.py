@@ -4,11 +4,16 @@ store the 6-component stress SGT, then the kernel time integral per pair
 code/srcF.kernel/SI_ker_sta.F90 per pair).
 
 Two ways to produce the same kernels:
-  plain    one solve per station, launch by launch, FP32 SGT
+  plain    one solve per station, launch by launch, FP32 SGT; with
+           host_sgt the SGT of each station leaves the GPU after its solve
+           and each pair's two SGTs come back for the integral (the
+           reference's SGT-file round trip, through host memory: at the
+           configured record lengths the FP32 SGT of all stations does not
+           fit in HBM beside the solver)
   batched  all stations in one batched solve (members share the model;
            step graphs; tile shape and batch width from the autotuner),
-           SGT stored as BF16 or FP16 codes with per-block scales and
-           restored on the device before the integral
+           SGT stored as BF16 or FP16 codes with per-block scales, kept in
+           HBM and restored on the device before the integral
 
 Both return the kernels (Kap, Kaq, Kbp, Kbq per pair, on the host) and a
 dict of wall times: setup (solver construction: metric, media, sources,
@@ -45,10 +50,14 @@ def _pair(RA, SB, dl, dm, dt, V, U, win, M0, filt):
     return [o.to_host() for o in out]
 
 
-def plain(model, stations, nsteps, pairs, tinv=10, step=2, V=None, U=None, win=None, M0=1e16, filt=None):
-    """One solve per station (FP32 SGT), then the pair kernels."""
+def plain(model, stations, nsteps, pairs, tinv=10, step=2, V=None, U=None, win=None, M0=1e16, filt=None,
+          host_sgt=False):
+    """One solve per station (FP32 SGT), then the pair kernels.  host_sgt:
+    each station's SGT goes to host memory after its solve (timed as
+    sgt_out) and each pair's two SGTs back to the GPU (part of kernel)."""
     sgts, solvers = {}, []
-    t_setup = t_steps = 0.0
+    t_setup = t_steps = t_out = 0.0
+    used = {x for p in pairs for x in p}
     for q, st in enumerate(stations):
         t0 = time.perf_counter()
         m, _ = models.virtual_sources(model, [st], [nsteps[q]])
@@ -60,23 +69,37 @@ def plain(model, stations, nsteps, pairs, tinv=10, step=2, V=None, U=None, win=N
         s.run(nsteps[q])
         s.sync()
         t_setup += t1 - t0
-        t_steps += time.perf_counter() - t1
-        ptr, nrec, npts = s.snapshot_device(0)
-        sgts[q] = (ptr, nrec, npts)
-        solvers.append(s)
+        t2 = time.perf_counter()
+        t_steps += t2 - t1
+        if host_sgt:
+            if q in used:
+                sgts[q] = s.snapshot(0)  # (nrec, 6, npts) on the host
+            s.close()
+            t_out += time.perf_counter() - t2
+        else:
+            ptr, nrec, npts = s.snapshot_device(0)
+            sgts[q] = (ptr, nrec, npts)
+            solvers.append(s)
     t0 = time.perf_counter()
     dl, dm = _media_on(model, snap)
     ks = []
     for a, b in pairs:
-        knt = min(sgts[a][1], sgts[b][1])
-        RA = kernel.snapshot_series(sgts[a][0], knt, 6, sgts[a][2])
-        SB = kernel.snapshot_series(sgts[b][0], knt, 6, sgts[b][2])
+        if host_sgt:
+            knt = min(sgts[a].shape[0], sgts[b].shape[0])
+            da, db = DeviceArray.from_host(sgts[a][:knt]), DeviceArray.from_host(sgts[b][:knt])
+            npts = sgts[a].shape[2]
+            RA = kernel.snapshot_series(da.ptr, knt, 6, npts)
+            SB = kernel.snapshot_series(db.ptr, knt, 6, npts)
+        else:
+            knt = min(sgts[a][1], sgts[b][1])
+            RA = kernel.snapshot_series(sgts[a][0], knt, 6, sgts[a][2])
+            SB = kernel.snapshot_series(sgts[b][0], knt, 6, sgts[b][2])
         ks.append(_pair(RA, SB, dl, dm, model.stept * tinv, V[:knt], U[:knt], win, M0, filt))
     device_sync()
     t_kernel = time.perf_counter() - t0
     for s in solvers:
         s.close()
-    return ks, dict(setup=t_setup, steps=t_steps, kernel=t_kernel)
+    return ks, dict(setup=t_setup, steps=t_steps, sgt_out=t_out, kernel=t_kernel)
 
 
 def batched(model, stations, nsteps, pairs, fmt="bf16", tinv=10, step=2, V=None, U=None, win=None, M0=1e16,
