@@ -17,7 +17,7 @@ FIELDS = ("Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Txy", "Txz", "Tyz")
 
 # every symbol declared in include/fwant.h
 SYMBOLS = (
-    "fw_create", "fw_destroy", "fw_last_error", "fw_dims", "fw_set_grid", "fw_build_metric",
+    "fw_create", "fw_destroy", "fw_last_error", "fw_dims", "fw_set_grid", "fw_build_metric", "fw_build_metric3d",
     "fw_set_media", "fw_set_force", "fw_set_moment", "fw_set_receivers", "fw_add_snapshot",
     "fw_init", "fw_get_pml_profiles", "fw_step", "fw_stage", "fw_run", "fw_reference_steps", "fw_sync",
     "fw_get_receivers", "fw_get_snapshot", "fw_snapshot_device", "fw_add_snapshot_packed",
@@ -65,6 +65,7 @@ def _sig(lib):
         "fw_dims": (C.c_int, [H, ip, ip, ip, ip]),
         "fw_set_grid": (C.c_int, [H, fp, fp, fp, fp, fp, fp, fp, fp]),
         "fw_build_metric": (C.c_int, [H, dp, dp, dp, fp, fp, fp, fp, fp]),
+        "fw_build_metric3d": (C.c_int, [H, dp, dp, dp, fp, fp, fp, fp]),
         "fw_set_media": (C.c_int, [H, fp, fp, fp]),
         "fw_set_force": (C.c_int, [H, C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, fp, fp, fp, fp]),
         "fw_set_moment": (C.c_int, [H, C.c_int, C.c_int, C.c_int, ip, fp, fp, fp, fp, fp]),

@@ -2,8 +2,8 @@
 
 The library is compiled from fwant_amd/csrc/*.hip with hipcc for gfx950 only,
 one object per translation unit, in parallel: the stage kernels
-(fw_stage_inst.hip) are compiled once per (tile shape, half of the F/B
-patterns).  Objects go
+(fw_stage_inst.hip) are compiled once per (grid kind, tile shape, half of
+the F/B patterns).  Objects go
 to fwant_amd/_build/, the library to fwant_amd/libfwant.so.
 Packed-FP32 instruction selection is disabled (-packed-fp32-ops).
 """
@@ -70,10 +70,12 @@ def build_fortran(verbose: bool = True) -> str:
 def _units():
     """(object path, source path, extra flags) of every translation unit."""
     u = [(os.path.join(BUILD, s.replace(".hip", ".o")), os.path.join(CSRC, s), []) for s in SOURCES]
-    for t in range(NTILE):
-        for half in range(2):
-            u.append((os.path.join(BUILD, f"fw_stage_{t}_{half}.o"), os.path.join(CSRC, STAGE_SRC),
-                      [f"-DFW_TILE={t}", f"-DFW_PHALF={half}"]))
+    for curv in range(2):
+        for t in range(NTILE):
+            for half in range(2):
+                tag = "c" if curv else ""
+                u.append((os.path.join(BUILD, f"fw_stage{tag}_{t}_{half}.o"), os.path.join(CSRC, STAGE_SRC),
+                          [f"-DFW_TILE={t}", f"-DFW_PHALF={half}", f"-DFW_CURV={curv}"]))
     return u
 
 

@@ -127,6 +127,30 @@ __host__ __device__ inline float stf_eval(int which, int id, float t, float t0, 
 }
 
 // ---------------------------------------------------------------------------
+// Mode C (terrain-following grid: theta = x(i), phi = y(j), r = r(i, j, k);
+// no counterpart in the reference, whose grid is a spherical shell).  At a
+// free-surface cell with slopes p, q (N = (p, q, 1)) the radial derivatives
+// a = zeta_r F(V) of the velocities follow from the traction-rate condition
+// sigma_t . N = S, which is linear in a:
+//   (mu n2 I + (lam + mu) N N^T) a = R,   n2 = N . N,
+//   R = S - sigma_t(a = 0) . N,
+// solved in closed form (Sherman-Morrison):
+//   a = (R - (lam + mu) / ((lam + 2 mu) n2) (N . R) N) / (mu n2).
+// The strain rates then take the zeta parts
+//   E11 += p aX, E22 += q aY, E33 = aZ, E12 += (p aY + q aX) / 2,
+//   E13 += (aX + p aZ) / 2, E23 += (aY + q aZ) / 2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void surf_solve(float lam, float miu, float p, float q, float R0, float R1, float R2,
+                                           float& aX, float& aY, float& aZ) {
+  const float n2 = 1.0f + p * p + q * q;
+  const float g = (lam + miu) / ((lam + 2.0f * miu) * n2) * (p * R0 + q * R1 + R2);
+  const float mn = miu * n2;
+  aX = (R0 - g * p) / mn;
+  aY = (R1 - g * q) / mn;
+  aZ = (R2 - g) / mn;
+}
+
+// ---------------------------------------------------------------------------
 // Reduced-precision SGT storage: a value x of a 16x16x16 block with largest
 // magnitude amax is stored as a 16-bit code of x/amax (BF16) or of
 // x*32768/amax (FP16), both rounded to nearest even; the block keeps the FP32

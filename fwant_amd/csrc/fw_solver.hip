@@ -74,7 +74,44 @@ struct SurfArgs {
   const int* __restrict__ clk;  // device step clock: clk[0] = ntime
   int ni1, ni2, nj1, nj2, nk2, ntw, stf_id, save;  // save: 0 none, 1 planes 0-2, 2 planes 3-5
   float tinc, stept;
+  // mode C: slopes p, q per cell (null: the spherical grid)
+  const float* __restrict__ cp;
+  const float* __restrict__ cq;
 };
+// Mode C imaging (no counterpart in the reference): the traction
+// combinations tau = sigma . N, N = (p, q, 1) (tau_z = Tzz + p Txz + q Tyz,
+// tau_x = Txz + p Txx + q Txy, tau_y = Tyz + p Txy + q Tyy) are set to the
+// source traction at k = nk2 by solving for Txz, Tyz, then Tzz, and the
+// planes above hold tau's odd images in Tzz Txz Tyz.  With p = q = 0 this is
+// free_timg.
+__device__ __forceinline__ void surface_curv(const SurfArgs& g, float* Tzz, float* Txz, float* Tyz, long c2, long cs,
+                                             float tx, float ty, float tz, const float* bzz, const float* bxz,
+                                             const float* byz, long slot) {
+  const float* Txx = g.win + slot * g.bstride + TXX * g.fs;
+  const float* Tyy = g.win + slot * g.bstride + TYY * g.fs;
+  const float* Txy = g.win + slot * g.bstride + TXY * g.fs;
+  float ixx[4], iyy[4], ixy[4], pp[4], qq[4];
+#pragma unroll
+  for (int k = 0; k <= 3; ++k) {
+    const long c = cs - k * g.sz;
+    ixx[k] = Txx[c]; iyy[k] = Tyy[c]; ixy[k] = Txy[c];
+    pp[k] = g.cp[c]; qq[k] = g.cq[c];
+  }
+#pragma unroll
+  for (int k = 1; k <= 3; ++k) {
+    const float p = pp[k], q = qq[k];
+    Tzz[cs + k * g.sz] = 2.0f * tz - (bzz[k] + p * bxz[k] + q * byz[k]);
+    Txz[cs + k * g.sz] = 2.0f * tx - (bxz[k] + p * ixx[k] + q * ixy[k]);
+    Tyz[cs + k * g.sz] = 2.0f * ty - (byz[k] + p * ixy[k] + q * iyy[k]);
+  }
+  const float p = pp[0], q = qq[0];
+  const float sxz = tx - p * ixx[0] - q * ixy[0];
+  const float syz = ty - p * ixy[0] - q * iyy[0];
+  Txz[cs] = sxz;
+  Tyz[cs] = syz;
+  Tzz[cs] = tz - p * sxz - q * syz;
+  (void)c2;
+}
 __global__ void k_surface(SurfArgs g) {
   const int i = g.ni1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = g.nj1 + blockIdx.y * blockDim.y + threadIdx.y;
@@ -112,6 +149,10 @@ __global__ void k_surface(SurfArgs g) {
     tx = g.F[q] / se.S * disp;
     ty = g.F[g.fblk + q] / se.S * disp;
     tz = g.F[2 * g.fblk + q] / se.S * disp;
+  }
+  if (g.cp) {
+    surface_curv(g, Tzz, Txz, Tyz, c2, cs, tx, ty, tz, bzz, bxz, byz, slot);
+    return;
   }
 #pragma unroll
   for (int k = 1; k <= 3; ++k) {
@@ -162,6 +203,9 @@ struct SrcArgs {
   const int* __restrict__ clk;      // device step clock: clk[0] = ntime
   int ncell, nact, kind;  // kind: the RK form (stage index)
   float tinc, stept, a, b;
+  // mode C: slopes p, q per cell (null: the spherical grid)
+  const float* __restrict__ cp;
+  const float* __restrict__ cq;
 };
 __global__ void k_src(SrcArgs g) {
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -202,12 +246,28 @@ __global__ void k_src(SrcArgs g) {
       const float vx = g.F[qq] * s.w * rate, vy = g.F[g.fblk + qq] * s.w * rate,
                   vz = g.F[2 * g.fblk + qq] * s.w * rate;
       const float lam = g.lam[c], miu = g.mu[c], lam2mu = lam + 2.0f * miu;
-      const float dE33 = vz / lam2mu, dE13 = vx / miu / 2.0f, dE23 = vy / miu / 2.0f;
-      dh[TXX] += lam * dE33;
-      dh[TYY] += lam * dE33;
-      dh[TZZ] += lam2mu * dE33;
-      dh[TXZ] += 2.0f * miu * dE13;
-      dh[TYZ] += 2.0f * miu * dE23;
+      const float p = g.cp ? g.cp[c] : 0.f, qs = g.cq ? g.cq[c] : 0.f;
+      if (p == 0.f && qs == 0.f) {
+        const float dE33 = vz / lam2mu, dE13 = vx / miu / 2.0f, dE23 = vy / miu / 2.0f;
+        dh[TXX] += lam * dE33;
+        dh[TYY] += lam * dE33;
+        dh[TZZ] += lam2mu * dE33;
+        dh[TXZ] += 2.0f * miu * dE13;
+        dh[TYZ] += 2.0f * miu * dE23;
+      } else {
+        // mode C: the traction rate's share of the surface solve (the
+        // stage kernel solved with a zero source; the solve is linear)
+        float aX, aY, aZ;
+        surf_solve(lam, miu, p, qs, vx, vy, vz, aX, aY, aZ);
+        const float E11 = p * aX, E22 = qs * aY, E33 = aZ, E12 = (p * aY + qs * aX) * 0.5f;
+        const float E13 = (aX + p * aZ) * 0.5f, E23 = (aY + qs * aZ) * 0.5f;
+        dh[TXX] += lam2mu * E11 + lam * (E22 + E33);
+        dh[TYY] += lam2mu * E22 + lam * (E11 + E33);
+        dh[TZZ] += lam2mu * E33 + lam * (E11 + E22);
+        dh[TXY] += 2.0f * miu * E12;
+        dh[TXZ] += 2.0f * miu * E13;
+        dh[TYZ] += 2.0f * miu * E23;
+      }
     }
   }
   float* wout = g.wout + slot * g.bstride;
@@ -367,6 +427,7 @@ __global__ __launch_bounds__(256) void k_snapshot_pack(PackArgs g) {
 // and the inverse of x_xi = (B.x + F.x)/2; ghost values copied from the edge.
 __global__ void k_metric(const double* __restrict__ g, int n, int lo, int hi, float* __restrict__ inv,
                          const double* __restrict__ gx_for_trig, float* __restrict__ xsin, float* __restrict__ xcot) {
+#pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (xsin) {
@@ -383,6 +444,80 @@ __global__ void k_metric(const double* __restrict__ g, int n, int lo, int hi, fl
     f += g[ic - 1 + q] * cF[q];
   }
   inv[i] = (float)(1.0 / ((b + f) * 0.5));
+}
+
+// Mode C metric (no counterpart in the reference): terrain-following grid
+// theta = x(i), phi = y(j), r = r(i, j, k) (dense FP64 input r).  FP64:
+//   zeta_r = 1 / r_zeta   (k_metric's expression along the column),
+//   p = -xi_x r_xi / r,  q = -eta_y r_eta / (r sin(theta)),
+// the xi / eta derivatives as 1/2 (F + B) with B written as the mirrored F
+// taps (exactly 0 where r is constant along the row); derivative values on
+// the 3 outer planes of each axis copied from the nearest plane the stencil
+// fits; 1/r of the FP32-rounded radius.  Outputs in the pitched field
+// layout (cell i + sy j + sz k).  Like k_metric, every FP64 operation is
+// rounded on its own (no contraction), so that a flat grid gives the 1-D
+// metric bit for bit.
+__device__ double inv_metric1d(const double* __restrict__ g, int i) {
+#pragma clang fp contract(off)
+  const double cF[5] = {-0.30874, -0.6326, 1.2330, -0.3334, 0.04168};
+  const double cB[5] = {-0.04168, 0.3334, -1.2330, 0.6326, 0.30874};
+  double b = 0.0, f = 0.0;
+  for (int q = 0; q < 5; ++q) {
+    b += g[i - 3 + q] * cB[q];
+    f += g[i - 1 + q] * cF[q];
+  }
+  return 1.0 / ((b + f) * 0.5);
+}
+__device__ double dmet_mirror(const double* __restrict__ g, long st) {
+#pragma clang fp contract(off)
+  const double cF[5] = {-0.30874, -0.6326, 1.2330, -0.3334, 0.04168};
+  double f = 0.0, b = 0.0;
+  for (int q = 0; q < 5; ++q) {
+    f += g[(q - 1) * st] * cF[q];
+    b -= g[(1 - q) * st] * cF[q];
+  }
+  return (f + b) * 0.5;
+}
+__global__ __launch_bounds__(256) void k_metric3d(const double* __restrict__ x, const double* __restrict__ y,
+                                                  const double* __restrict__ r, int nx, int ny, int nz, long sy, long sz,
+                                                  float* __restrict__ p, float* __restrict__ q, float* __restrict__ zz,
+                                                  float* __restrict__ rr) {
+#pragma clang fp contract(off)
+  const long u = (long)blockIdx.x * 256 + threadIdx.x;
+  if (u >= (long)nx * ny * nz) return;
+  const int i = (int)(u % nx), j = (int)((u / nx) % ny), k = (int)(u / ((long)nx * ny));
+  const long dsy = nx, dsz = (long)nx * ny;
+  const int kc = min(max(k, 3), nz - 4), ic = min(max(i, 3), nx - 4), jc = min(max(j, 3), ny - 4);
+  const double cF[5] = {-0.30874, -0.6326, 1.2330, -0.3334, 0.04168};
+  const double cB[5] = {-0.04168, 0.3334, -1.2330, 0.6326, 0.30874};
+  const double* col = r + i + dsy * j;
+  double b = 0.0, f = 0.0;
+  for (int m = 0; m < 5; ++m) {
+    b += col[(kc - 3 + m) * dsz] * cB[m];
+    f += col[(kc - 1 + m) * dsz] * cF[m];
+  }
+  const long c = i + sy * j + sz * k;
+  zz[c] = (float)(1.0 / ((b + f) * 0.5));
+  const long cx = ic + dsy * j + dsz * k, cy = i + dsy * jc + dsz * k;
+  p[c] = (float)(-inv_metric1d(x, ic) * dmet_mirror(r + cx, 1) / r[cx]);
+  q[c] = (float)(-inv_metric1d(y, jc) * dmet_mirror(r + cy, dsy) / (r[cy] * sin(x[i])));
+  rr[c] = (float)(1.0 / (double)(float)r[u]);
+}
+// one-sided zeta derivatives F(p), F(q) (fwd) or B (bwd) of the slopes, FP32,
+// on planes 3 .. nz-4 (0 elsewhere)
+__global__ __launch_bounds__(256) void k_metric3d_dz(const float* __restrict__ p, const float* __restrict__ q, int nx,
+                                                     int ny, int nz, long sy, long sz, float* __restrict__ dpF,
+                                                     float* __restrict__ dqF, float* __restrict__ dpB,
+                                                     float* __restrict__ dqB) {
+  const long u = (long)blockIdx.x * 256 + threadIdx.x;
+  if (u >= (long)nx * ny * nz) return;
+  const int i = (int)(u % nx), j = (int)((u / nx) % ny), k = (int)(u / ((long)nx * ny));
+  const long c = i + sy * j + sz * k;
+  const bool in = k >= 3 && k <= nz - 4;
+  dpF[c] = in ? dop<1>(p + c, sz) : 0.f;
+  dqF[c] = in ? dop<1>(q + c, sz) : 0.f;
+  dpB[c] = in ? dop<0>(p + c, sz) : 0.f;
+  dqB[c] = in ? dop<0>(q + c, sz) : 0.f;
 }
 
 // grid-stride fill: a dispatch holds fewer than 2^32 work-items, so a batch's
@@ -446,6 +581,12 @@ struct fw_solver_s {
   int4* d_items = nullptr;    // stage work items: PML-shell items, then interior items
   int n_narrow = 0, n_lean = 0, n_pml = 0;
   int xg = 1;  // stage-kernel items per XCD run
+  // mode C (terrain-following grid): 8 volumes in the field layout:
+  // p, q, zeta_r, 1/r, then F(p), F(q), B(p), B(q) along zeta
+  bool curv = false;
+  float* d_met3 = nullptr;
+  float* met3[8]{};
+  std::vector<float> hr3;  // radius of every node (dense), for the source weights
   // host copies
   std::vector<float> hx, hy, hz, hxsin, hxcot, hxi, het, hze;
   std::vector<float> hlam, hmu, hrho;
@@ -562,6 +703,7 @@ int fw_destroy(fw_handle h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   hipFree(h->d_fluidcol);
   hipFree(h->d_sets); hipFree(h->d_media); hipFree(h->d_1d); hipFree(h->d_msurf); hipFree(h->d_pml);
+  hipFree(h->d_met3);
   hipFree(h->d_stage);
   for (auto& f : h->face) hipFree(f.d);
   for (void* p : h->d_src_allocs) hipFree(p);
@@ -702,6 +844,50 @@ int fw_set_media(fw_handle h, const float* lam, const float* mu, const float* rh
   if ((rc = upload_volume(h, h->rrho, rr.data()))) return rc;
   HIPCHK(hipStreamSynchronize(h->stream));
   h->have_media = true;
+  h->ready = false;
+  return FW_OK;
+}
+
+int fw_build_metric3d(fw_handle h, const double* x, const double* y, const double* r, float* op, float* oq, float* ozz,
+                      float* orr) {
+  if (!h || !x || !y || !r) return fail(h, FW_ERR_ARG, "fw_build_metric3d: null argument");
+  if (!h->have_grid) return fail(h, FW_ERR_STATE, "fw_build_metric3d: set the 1-D grid first (fw_set_grid / fw_build_metric)");
+  const int nx = h->nx, ny = h->ny, nz = h->nz;
+  const size_t n = (size_t)nx * ny * nz;
+  for (size_t u = 0; u < n; ++u)
+    if (!(r[u] > 0.0)) return fail(h, FW_ERR_ARG, "fw_build_metric3d: radii must be > 0");
+  for (int i = 0; i < nx; ++i)
+    if (!(std::sin(x[i]) != 0.0)) return fail(h, FW_ERR_ARG, "fw_build_metric3d: sin(x) must be nonzero");
+  if (!h->d_met3) {
+    HIPCHK(hipMalloc(&h->d_met3, sizeof(float) * (8 * (size_t)h->fs + 64)));
+    for (int v = 0; v < 8; ++v) h->met3[v] = h->d_met3 + h->off + (size_t)v * h->fs;
+  }
+  double* d = nullptr;
+  HIPCHK(hipMalloc(&d, sizeof(double) * (n + nx + ny)));
+  hipError_t e = hipMemcpyAsync(d, r, sizeof(double) * n, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + n, x, sizeof(double) * nx, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + n + nx, y, sizeof(double) * ny, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) {
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    float** m = h->met3;
+    hipLaunchKernelGGL(k_metric3d, dim3(nb), dim3(256), 0, h->stream, d + n, d + n + nx, d, nx, ny, nz, h->sy, h->sz,
+                       m[0], m[1], m[2], m[3]);
+    hipLaunchKernelGGL(k_metric3d_dz, dim3(nb), dim3(256), 0, h->stream, m[0], m[1], nx, ny, nz, h->sy, h->sz, m[4],
+                       m[5], m[6], m[7]);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(d);
+  if (e != hipSuccess) return fail(h, FW_ERR_HIP, std::string("fw_build_metric3d: ") + hipGetErrorString(e));
+  float* outs[4] = {op, oq, ozz, orr};
+  for (int v = 0; v < 4; ++v)
+    if (outs[v]) {
+      int rc = download_volume(h, outs[v], h->met3[v]);
+      if (rc) return rc;
+    }
+  h->hr3.resize(n);
+  for (size_t u = 0; u < n; ++u) h->hr3[u] = (float)r[u];
+  h->curv = true;
   h->ready = false;
   return FW_OK;
 }
@@ -894,8 +1080,12 @@ static int build_sources(fw_handle h) {
     return 0;
   };
   const std::vector<float>&x = h->hx, &y = h->hy, &z = h->hz;
+  // mode C: the radial extent along the column and the radius of the cell
+  auto rad = [&](int i, int j, int k) { return h->curv ? h->hr3[i + (size_t)h->nx * (j + (size_t)h->ny * k)] : z[k]; };
   auto volume = [&](int i, int j, int k) {
-    return (x[i + 1] - x[i - 1]) * (y[j + 1] - y[j - 1]) * (z[k + 1] - z[k - 1]) / 8.0f * (z[k] * z[k]) * h->hxsin[i];
+    const float rk = rad(i, j, k);
+    return (x[i + 1] - x[i - 1]) * (y[j + 1] - y[j - 1]) * (rad(i, j, k + 1) - rad(i, j, k - 1)) / 8.0f * (rk * rk) *
+           h->hxsin[i];
   };
   auto norm_delt = [&](float* d, float x0, float y0, float z0, int sk) {
     const float r0 = 1.5f;
@@ -926,7 +1116,10 @@ static int build_sources(fw_handle h) {
     const int si = h->findx[3 * n] - 1, sj = h->findx[3 * n + 1] - 1, sk = h->findx[3 * n + 2] - 1;
     // surface traction entries (src_surface), last (n, m) wins
     if (h->freenode && si >= h->ni1 && si <= h->ni2 && sj >= h->nj1 && sj <= h->nj2 && sk == h->nk2) {
-      const float S = (x[si + 1] - x[si - 1]) * (y[sj + 1] - y[sj - 1]) / 4.0f * (z[sk] * z[sk]) * h->hxsin[si];
+      // projected area at the surface radius (mode C: sigma . N = F / S, N's
+      // length cancels the slope's area)
+      const float zs = rad(si, sj, sk);
+      const float S = (x[si + 1] - x[si - 1]) * (y[sj + 1] - y[sj - 1]) / 4.0f * (zs * zs) * h->hxsin[si];
       const long c2 = si + h->sy * (long)sj + h->ps * (long)fslot(n);
       auto it = surf_of.find(c2);
       SurfEntry e{n, h->fntw - 1, 0.f, S};
@@ -1119,6 +1312,42 @@ int fw_init(fw_handle h) {
     if (f.cnt >= (1L << 31)) return fail(h, FW_ERR_ARG, "CFS-PML slab with 2^31 or more cells");
     HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt * h->nbatch));
   }
+  if (h->curv) {
+    // mode C: the CFS-PML corrections are the spherical scheme's, so every
+    // slab cell must sit on a flat stretch of grid (slopes 0, radius z(k));
+    // fluid cells are not supported on a terrain-following grid
+    // (a cell whose +-3 box is flat has p = q = 0, F(p) = F(q) = 0, the 1-D
+    // zeta_r and 1/z: the non-flat cells are dilated by 3 along each axis)
+    const int nx = h->nx, ny = h->ny, nz = h->nz;
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<uint8_t> bad(n), tmp(n);
+    for (int k = 0; k < nz; ++k)
+      for (size_t u = (size_t)nx * ny * k; u < (size_t)nx * ny * (k + 1); ++u) bad[u] = h->hr3[u] != h->hz[k];
+    const long st[3] = {1, nx, (long)nx * ny};
+    const int ext[3] = {nx, ny, nz};
+    for (int ax = 0; ax < 3; ++ax) {
+      for (size_t u = 0; u < n; ++u) {
+        const int c = (int)((u / st[ax]) % ext[ax]);
+        uint8_t b = 0;
+        for (int d = -3; d <= 3; ++d)
+          if (c + d >= 0 && c + d < ext[ax]) b |= bad[u + d * st[ax]];
+        tmp[u] = b;
+      }
+      bad.swap(tmp);
+    }
+    for (int n6 = 0; n6 < 6; ++n6) {
+      const PmlFace& f = h->face[n6];
+      if (!f.on) continue;
+      for (int k = f.lo[2]; k <= f.hi[2]; ++k)
+        for (int j = f.lo[1]; j <= f.hi[1]; ++j)
+          for (int i = f.lo[0]; i <= f.hi[0]; ++i)
+            if (bad[i + (size_t)nx * (j + (size_t)ny * k)])
+              return fail(h, FW_ERR_ARG, "fw_init: mode C needs a flat grid (r = z(k)) on every CFS-PML slab cell "
+                                         "and 3 cells beyond");
+    }
+    for (size_t u = 0; u < n; ++u)
+      if (h->hmu[u] <= SEIS_ZERO) return fail(h, FW_ERR_ARG, "fw_init: mode C does not support fluid cells (mu <= 0)");
+  }
   int rc = build_items(h);
   if (rc) return rc;
   rc = build_sources(h);
@@ -1147,10 +1376,12 @@ int fw_get_pml_profiles(fw_handle h, float* A, float* B, float* D) {
 // Kernel dispatch over the tile shapes, 8 F/B patterns and 4 RK forms (the
 // instantiations live in fw_stage_inst.hip).
 // ---------------------------------------------------------------------------
-static StageFn stage_table(int c, int p, int form) {
+static StageFn stage_table(int c, int p, int form, bool curv) {
   static StageFn (*const T[NTILECFG][2])(int, int) = {{stage_fn_0_0, stage_fn_0_1}, {stage_fn_1_0, stage_fn_1_1},
                                                        {stage_fn_2_0, stage_fn_2_1}, {stage_fn_3_0, stage_fn_3_1}};
-  return T[c][(p >> 2) & 1](p, form);
+  static StageFn (*const TC[NTILECFG][2])(int, int) = {{stage_fnc_0_0, stage_fnc_0_1}, {stage_fnc_1_0, stage_fnc_1_1},
+                                                        {stage_fnc_2_0, stage_fnc_2_1}, {stage_fnc_3_0, stage_fnc_3_1}};
+  return (curv ? TC : T)[c][(p >> 2) & 1](p, form);
 }
 
 // Work items of the stage kernels: tiles over chunks of k planes in three
@@ -1283,6 +1514,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) 
     s.bstride = h->bstride; s.clk = h->d_clk;
     s.ni1 = h->ni1; s.ni2 = h->ni2; s.nj1 = h->nj1; s.nj2 = h->nj2; s.nk2 = h->nk2;
     s.ntw = h->fntw; s.stf_id = h->fstf; s.save = st == 0 ? 1 : (st == 2 ? 0 : 2); s.tinc = tinc; s.stept = h->cfg.stept;
+    s.cp = h->curv ? h->met3[0] : nullptr; s.cq = h->curv ? h->met3[1] : nullptr;
     dim3 blk(64, 4), grd((h->cfg.ni + 63) / 64, (h->cfg.nj + 3) / 4, nact);
     hipLaunchKernelGGL(k_surface, grd, blk, 0, h->stream, s);
   }
@@ -1299,6 +1531,11 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) 
   g.n1 = h->nx + h->ny + h->nz;
   g.pb1 = h->d_pml; g.pad = h->d_pml + g.n1; g.pdd = h->d_pml + 2 * g.n1;
   g.setin = IN[st]; g.setout = OUT[st];
+  if (h->curv) {
+    g.cp = h->met3[0]; g.cq = h->met3[1]; g.czz = h->met3[2]; g.crr = h->met3[3];
+    const bool zf = (pat & 1) != 0;
+    g.cdp = h->met3[zf ? 4 : 6]; g.cdq = h->met3[zf ? 5 : 7];
+  }
   g.pml = 0;
   for (int n = 0; n < 6; ++n) {
     const PmlFace& f = h->face[n];
@@ -1326,7 +1563,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) 
     g.slot0 = s0;
     g.nm = std::min(W, nact - s0);
     const dim3 grid(g.nitems * g.nm);
-    hipLaunchKernelGGL(stage_table(tc, pat, st), grid, sblk, 0, h->stream, g);
+    hipLaunchKernelGGL(stage_table(tc, pat, st, h->curv), grid, sblk, 0, h->stream, g);
   }
   if (h->timing) {
     hipEventRecord(h->ev[2 * h->nstage + 1], h->stream);
@@ -1344,6 +1581,7 @@ static int launch_stage(fw_handle h, int ntime, int st, int nact, int pat = -1) 
     s.fs = h->fs; s.bstride = h->bstride; s.rowslot = h->d_srow; s.clk = h->d_clk;
     s.ncell = h->n_srccell; s.nact = nact; s.kind = st;
     s.tinc = tinc; s.stept = h->cfg.stept; s.a = a; s.b = b;
+    s.cp = h->curv ? h->met3[0] : nullptr; s.cq = h->curv ? h->met3[1] : nullptr;
     hipLaunchKernelGGL(k_src, dim3((h->n_srccell + 127) / 128), dim3(128), 0, h->stream, s);
   }
   return FW_OK;

@@ -128,6 +128,15 @@ struct StageArgs {
   long bstride;
   int slot0, nm;
   int xg;  // items per XCD run (half the wide tiles of a row)
+  // mode C (terrain-following grid, CURV instantiations): slopes p, q,
+  // zeta_r, 1/r and the stage's one-sided zeta derivatives of p and q, per
+  // cell in the field layout
+  const float* __restrict__ cp;
+  const float* __restrict__ cq;
+  const float* __restrict__ czz;
+  const float* __restrict__ crr;
+  const float* __restrict__ cdp;
+  const float* __restrict__ cdq;
 };
 
 // Blocks are dealt round-robin to the 8 XCDs.  The (work item, member)
@@ -477,6 +486,26 @@ __device__ __forceinline__ float ld(const float* __restrict__ p, const HaloSlot&
   return ldv(p, h.off + kz, h.ok);
 }
 
+// Mode C: the traction combinations across the surface zeta = const of a
+// cell at offset a on plane kk, tau = sigma . N with N = (p, q, 1):
+//   tau_z = Tzz + p Txz + q Tyz, tau_x = Txz + p Txx + q Txy, tau_y = Tyz + p Txy + q Tyy;
+// above a free surface the imaged planes hold tau itself in Tzz Txz Tyz.
+__device__ __forceinline__ void tau_at(const StageArgs& g, const float* __restrict__ pTxx, const float* __restrict__ pTyy,
+                                       const float* __restrict__ pTxy, const float* __restrict__ pTzz,
+                                       const float* __restrict__ pTxz, const float* __restrict__ pTyz, unsigned a, bool ok,
+                                       int kk, float& tz, float& tx, float& ty) {
+  const float xx = ldv(pTxx, a, ok), yy = ldv(pTyy, a, ok), xy = ldv(pTxy, a, ok);
+  const float zz = ldv(pTzz, a, ok), xz = ldv(pTxz, a, ok), yz = ldv(pTyz, a, ok);
+  const float p = ldv(g.cp, a, ok), q = ldv(g.cq, a, ok);
+  if (g.freenode && kk > g.nk2) {
+    tz = zz; tx = xz; ty = yz;
+  } else {
+    tz = zz + p * xz + q * yz;
+    tx = xz + p * xx + q * xy;
+    ty = yz + p * xy + q * yy;
+  }
+}
+
 // LDS planes of the two groups (per buffer): velocity group Txx Txy Txz Tyy
 // Tyz (with halos) and Tzz (centres only); stress group Vx Vy Vz.
 template <int BYT, int RY, int TW>
@@ -491,8 +520,12 @@ enum { LTXX = 0, LTXY, LTXZ, LTYY, LTYZ, LTZZ };
 // (mod_macdrp.F90:928-933), fluid zeroing of shear-stress derivatives
 // (:917-926), RK update of Vx Vy Vz.  z window: Tzz Txz Tyz.
 // own: this group's LDS (VNL planes x 2 buffers), other: the stress group's.
+// CURV (mode C, terrain-following grid): the z window holds the traction
+// combinations tau_z tau_x tau_y (tau_at), the zeta part of row i of the
+// stress divergence is zeta_r (F(tau_i) - sigma_ix F(p) - sigma_iy F(q)),
+// 1/r is per cell; the plane centres of Tzz Txz Tyz are loaded separately.
 // ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML>
+template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML, bool CURV>
 __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
   // RS: rows per row group (the stride between a thread's rows)
   using G = TileGeom<BYT, RY, TW>;
@@ -527,8 +560,9 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
   const HaloSlot hx = xhalo_slot<XF, TW, TY, LW>(g, i0, j0, ie, je, tid);
   const HaloSlot hy = yhalo_slot<YF, TW, TY, LW>(g, i0, j0, ie, je, tid);
 
-  float w[RY][3][5];  // Tzz Txz Tyz
+  float w[RY][3][5];  // Tzz Txz Tyz (CURV: tau_z tau_x tau_y)
   float cTxx[RY], cTyy[RY], cTxy[RY];
+  float cTzz[RY], cTxz[RY], cTyz[RY];  // CURV: plane centres of the window fields
   bool ok[RY], comp[RY], fluid[RY];
   unsigned cc[RY];
   float et[RY];
@@ -544,13 +578,23 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
       const int o = ZF ? s - 1 : s - 3;
-      w[q][0][s] = ldv(pTzz, c0 + o * (int)sz, ok[q]);
-      w[q][1][s] = ldv(pTxz, c0 + o * (int)sz, ok[q]);
-      w[q][2][s] = ldv(pTyz, c0 + o * (int)sz, ok[q]);
+      if constexpr (CURV) {
+        tau_at(g, pTxx, pTyy, pTxy, pTzz, pTxz, pTyz, c0 + o * (int)sz, ok[q], k0 + o, w[q][0][s], w[q][1][s],
+               w[q][2][s]);
+      } else {
+        w[q][0][s] = ldv(pTzz, c0 + o * (int)sz, ok[q]);
+        w[q][1][s] = ldv(pTxz, c0 + o * (int)sz, ok[q]);
+        w[q][2][s] = ldv(pTyz, c0 + o * (int)sz, ok[q]);
+      }
     }
     cTxx[q] = ldv(pTxx, c0, ok[q]);
     cTyy[q] = ldv(pTyy, c0, ok[q]);
     cTxy[q] = ldv(pTxy, c0, ok[q]);
+    if constexpr (CURV) {
+      cTzz[q] = ldv(pTzz, c0, ok[q]);
+      cTxz[q] = ldv(pTxz, c0, ok[q]);
+      cTyz[q] = ldv(pTyz, c0, ok[q]);
+    }
   }
   AuxPre pre[RY];
 #pragma unroll
@@ -591,9 +635,9 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
       sTxx[o] = cTxx[q];
       sTxy[o] = cTxy[q];
       sTyy[o] = cTyy[q];
-      sTxz[o] = zc<ZF>(w[q][1]);
-      sTyz[o] = zc<ZF>(w[q][2]);
-      sTzz[o] = zc<ZF>(w[q][0]);
+      sTxz[o] = CURV ? cTxz[q] : zc<ZF>(w[q][1]);
+      sTyz[o] = CURV ? cTyz[q] : zc<ZF>(w[q][2]);
+      sTzz[o] = CURV ? cTzz[q] : zc<ZF>(w[q][0]);
     }
     {
       float* const px = hx.lds >= 0 ? s + hx.lds : spare;
@@ -630,13 +674,22 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
     }
     // prefetch plane k+1 (the last plane re-reads plane k: every address valid)
     float nTxx[RY], nTyy[RY], nTxy[RY], lead[RY][3];
+    float nTzz[RY], nTxz[RY], nTyz[RY];
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const unsigned c = cc[q] + kn;
       const unsigned l = c + (ZF ? 3u : 1u) * sz;
-      lead[q][0] = ldv(pTzz, l, ok[q]);
-      lead[q][1] = ldv(pTxz, l, ok[q]);
-      lead[q][2] = ldv(pTyz, l, ok[q]);
+      if constexpr (CURV) {
+        tau_at(g, pTxx, pTyy, pTxy, pTzz, pTxz, pTyz, l, ok[q], (more ? k + 1 : k) + (ZF ? 3 : 1), lead[q][0],
+               lead[q][1], lead[q][2]);
+        nTzz[q] = ldv(pTzz, c, ok[q]);
+        nTxz[q] = ldv(pTxz, c, ok[q]);
+        nTyz[q] = ldv(pTyz, c, ok[q]);
+      } else {
+        lead[q][0] = ldv(pTzz, l, ok[q]);
+        lead[q][1] = ldv(pTxz, l, ok[q]);
+        lead[q][2] = ldv(pTyz, l, ok[q]);
+      }
       nTxx[q] = ldv(pTxx, c, ok[q]);
       nTyy[q] = ldv(pTyy, c, ok[q]);
       nTxy[q] = ldv(pTxy, c, ok[q]);
@@ -649,7 +702,18 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
       for (int q = 0; q < RY; ++q) aux_plane<FORM, FULLV>(g, pre[q], slot, i, j0 + ty + RS * q, k, k0, 3, comp[q], pv[q]);
     }
     // plane metric with the plane's other loads (used right after the barrier)
-    const float ze = g.zeta_z[k], rzk = g.rz[k];
+    const float ze1 = g.zeta_z[k], rz1 = g.rz[k];
+    float mze[RY], mrz[RY], mdp[RY], mdq[RY];  // CURV: zeta_r, 1/r, F(p), F(q) of the cell
+    if constexpr (CURV) {
+#pragma unroll
+      for (int q = 0; q < RY; ++q) {
+        const unsigned c = cc[q] + kz;
+        mze[q] = ldv(g.czz, c, ok[q]);
+        mrz[q] = ldv(g.crr, c, ok[q]);
+        mdp[q] = ldv(g.cdp, c, ok[q]);
+        mdq[q] = ldv(g.cdq, c, ok[q]);
+      }
+    }
     __syncthreads();
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
@@ -657,22 +721,33 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
       if (comp[q]) {
         const unsigned c = cc[q] + kz;
         const int o = (ty + RS * q + 3) * LW + tx + 3;
+        const float ze = CURV ? mze[q] : ze1, rzk = CURV ? mrz[q] : rz1;
         const float DxTxx = dlds<XF>(sTxx + o, 1) * xi;
         float DxTxy = dlds<XF>(sTxy + o, 1) * xi;
         float DxTxz = dlds<XF>(sTxz + o, 1) * xi;
         const float DyTyy = dlds<YF>(sTyy + o, LW) * et[q];
         float DyTxy = dlds<YF>(sTxy + o, LW) * et[q];
         float DyTyz = dlds<YF>(sTyz + o, LW) * et[q];
-        const float DzTzz = dz5<ZF>(w[q][0]) * ze;
-        float DzTxz = dz5<ZF>(w[q][1]) * ze;
-        float DzTyz = dz5<ZF>(w[q][2]) * ze;
+        const float Txx = cTxx[q], Tyy = cTyy[q], Txy = cTxy[q];
+        const float Tzz = CURV ? cTzz[q] : zc<ZF>(w[q][0]), Txz = CURV ? cTxz[q] : zc<ZF>(w[q][1]),
+                    Tyz = CURV ? cTyz[q] : zc<ZF>(w[q][2]);
+        float DzTzz0, DzTxz0, DzTyz0;
+        if constexpr (CURV) {
+          DzTzz0 = (dz5<ZF>(w[q][0]) - Txz * mdp[q] - Tyz * mdq[q]) * ze;
+          DzTxz0 = (dz5<ZF>(w[q][1]) - Txx * mdp[q] - Txy * mdq[q]) * ze;
+          DzTyz0 = (dz5<ZF>(w[q][2]) - Txy * mdp[q] - Tyy * mdq[q]) * ze;
+        } else {
+          DzTzz0 = dz5<ZF>(w[q][0]) * ze;
+          DzTxz0 = dz5<ZF>(w[q][1]) * ze;
+          DzTyz0 = dz5<ZF>(w[q][2]) * ze;
+        }
+        const float DzTzz = DzTzz0;
+        float DzTxz = DzTxz0, DzTyz = DzTyz0;
         const float rDxTxy = DxTxy, rDxTxz = DxTxz, rDyTxy = DyTxy, rDyTyz = DyTyz, rDzTxz = DzTxz, rDzTyz = DzTyz;
         // fluid cells (mu <= SEIS_ZERO) occur only in the rare columns that hold one
         if (fluid[q] && g.mu[c] <= SEIS_ZERO) {
           DxTxy = 0.f; DxTxz = 0.f; DyTxy = 0.f; DyTyz = 0.f; DzTxz = 0.f; DzTyz = 0.f;
         }
-        const float Txx = cTxx[q], Tyy = cTyy[q], Txy = cTxy[q];
-        const float Tzz = zc<ZF>(w[q][0]), Txz = zc<ZF>(w[q][1]), Tyz = zc<ZF>(w[q][2]);
         float h[3];
         h[0] = rr[q] * (DxTxx * rzk + DyTxy * rzk * rxs + DzTxz + (3.0f * Txz + Txx * xc - Tyy * xc) * rzk);
         h[1] = rr[q] * (DxTxy * rzk + DyTyy * rzk * rxs + DzTyz + (2.0f * Txy * xc + 3.0f * Tyz) * rzk);
@@ -706,6 +781,11 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
       cTxx[q] = nTxx[q];
       cTyy[q] = nTyy[q];
       cTxy[q] = nTxy[q];
+      if constexpr (CURV) {
+        cTzz[q] = nTzz[q];
+        cTxz[q] = nTxz[q];
+        cTyz[q] = nTyz[q];
+      }
     }
     hxTxx = nhxTxx; hxTxy = nhxTxy; hxTxz = nhxTxz;
     hyTxy = nhyTxy; hyTyy = nhyTyy; hyTyz = nhyTyz;
@@ -721,8 +801,12 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 // their W_1 / W_3 the pre-image inputs saved at forms 1 / 3.
 // z window: Vx Vy Vz.  own: this group's LDS (SNL planes x 2 buffers),
 // other: the velocity group's.
+// CURV (mode C): with a = zeta_r F(V) the strain rates take the slope terms
+//   E11 += p aX, E22 += q aY, E12 += (p aY + q aX)/2, E13 += p aZ/2, E23 += q aZ/2,
+// and at a free-surface cell with p or q nonzero a comes from the
+// traction-rate condition (surf_solve; the source part is added by k_src).
 // ---------------------------------------------------------------------------
-template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML>
+template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML, bool CURV>
 __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
   using G = TileGeom<BYT, RY, TW>;
   constexpr int RS = BYT * (TX / TW), TY = G::TY, LW = G::LW, PL = G::PL;
@@ -859,7 +943,18 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
       for (int q = 0; q < RY; ++q) aux_plane<FORM, FULLS>(g, pre[q], slot, i, j0 + ty + RS * q, k, k0, 0, comp[q], pv[q]);
     }
     // plane metric with the plane's other loads (used right after the barrier)
-    const float ze = g.zeta_z[k], rzk = g.rz[k];
+    const float ze1 = g.zeta_z[k], rz1 = g.rz[k];
+    float mze[RY], mrz[RY], mp[RY], mq[RY];  // CURV: zeta_r, 1/r, slopes of the cell
+    if constexpr (CURV) {
+#pragma unroll
+      for (int q = 0; q < RY; ++q) {
+        const unsigned c = cc[q] + kz;
+        mze[q] = ldv(g.czz, c, ok[q]);
+        mrz[q] = ldv(g.crr, c, ok[q]);
+        mp[q] = ldv(g.cp, c, ok[q]);
+        mq[q] = ldv(g.cq, c, ok[q]);
+      }
+    }
     __syncthreads();
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
@@ -867,6 +962,7 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
       if (comp[q]) {
         const unsigned c = cc[q] + kz;
         const int o = (ty + RS * q + 3) * LW + tx + 3;
+        const float ze = CURV ? mze[q] : ze1, rzk = CURV ? mrz[q] : rz1;
         const float DxVx = dlds<XF>(s + o, 1) * xi;
         const float DxVy = dlds<XF>(s + PL + o, 1) * xi;
         const float DxVz = dlds<XF>(s + 2 * PL + o, 1) * xi;
@@ -882,16 +978,52 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
         }
         const float Vx = zc<ZF>(w[q][0]), Vy = zc<ZF>(w[q][1]), Vz = zc<ZF>(w[q][2]);
         const float lam = la[q], miu = mu_[q], lam2mu = lam + 2.0f * miu;
-        const float E11 = (DxVx + Vz) * rzk;
-        const float E22 = (Vx * xc + DyVy * rxs + Vz) * rzk;
-        const float E12 = (DxVy + DyVx * rxs - Vy * xc) * rzk * 0.5f;
-        float E33 = DzVz;
-        float E13 = (DxVz * rzk + DzVx - Vx * rzk) * 0.5f;
-        float E23 = (DyVz * rzk * rxs + DzVy - Vy * rzk) * 0.5f;
-        if (surf) {
-          E33 = -lam / lam2mu * (E11 + E22);
-          E13 = 0.f;
-          E23 = 0.f;
+        float E11, E22, E12, E33, E13, E23;
+        if constexpr (CURV) {
+          const float p = mp[q], qs = mq[q];
+          if (surf && !(p == 0.f && qs == 0.f)) {
+            // the stress rates without the zeta parts, then a from the
+            // traction-rate condition with a zero source
+            const float H11 = (DxVx + Vz) * rzk, H22 = (Vx * xc + DyVy * rxs + Vz) * rzk;
+            const float H12 = (DxVy + DyVx * rxs - Vy * xc) * rzk * 0.5f;
+            const float H13 = (DxVz * rzk - Vx * rzk) * 0.5f, H23 = (DyVz * rzk * rxs - Vy * rzk) * 0.5f;
+            const float tr = H11 + H22;
+            const float sxx = lam * tr + 2.0f * miu * H11, syy = lam * tr + 2.0f * miu * H22, szz = lam * tr;
+            const float sxy = 2.0f * miu * H12, sxz = 2.0f * miu * H13, syz = 2.0f * miu * H23;
+            float aX, aY, aZ;
+            surf_solve(lam, miu, p, qs, -(p * sxx + qs * sxy + sxz), -(p * sxy + qs * syy + syz),
+                       -(p * sxz + qs * syz + szz), aX, aY, aZ);
+            E11 = H11 + p * aX;
+            E22 = H22 + qs * aY;
+            E33 = aZ;
+            E12 = H12 + (p * aY + qs * aX) * 0.5f;
+            E13 = H13 + (aX + p * aZ) * 0.5f;
+            E23 = H23 + (aY + qs * aZ) * 0.5f;
+          } else {
+            E11 = (DxVx + Vz) * rzk + p * DzVx;
+            E22 = (Vx * xc + DyVy * rxs + Vz) * rzk + qs * DzVy;
+            E12 = ((DxVy + DyVx * rxs - Vy * xc) * rzk + p * DzVy + qs * DzVx) * 0.5f;
+            E33 = DzVz;
+            E13 = (DxVz * rzk + DzVx - Vx * rzk + p * DzVz) * 0.5f;
+            E23 = (DyVz * rzk * rxs + DzVy - Vy * rzk + qs * DzVz) * 0.5f;
+            if (surf) {
+              E33 = -lam / lam2mu * (E11 + E22);
+              E13 = 0.f;
+              E23 = 0.f;
+            }
+          }
+        } else {
+          E11 = (DxVx + Vz) * rzk;
+          E22 = (Vx * xc + DyVy * rxs + Vz) * rzk;
+          E12 = (DxVy + DyVx * rxs - Vy * xc) * rzk * 0.5f;
+          E33 = DzVz;
+          E13 = (DxVz * rzk + DzVx - Vx * rzk) * 0.5f;
+          E23 = (DyVz * rzk * rxs + DzVy - Vy * rzk) * 0.5f;
+          if (surf) {
+            E33 = -lam / lam2mu * (E11 + E22);
+            E13 = 0.f;
+            E23 = 0.f;
+          }
         }
         float h[6];  // Txx Tyy Tzz Txy Txz Tyz
         h[0] = lam2mu * E11 + lam * (E22 + E33);
@@ -960,7 +1092,7 @@ struct StageLds {
 // corrections), the wide PML-shell tiles the PML body, the wide interior
 // tiles the lean body.  At most 128 VGPRs (4 waves per SIMD: two 8-wave
 // blocks per CU).
-template <int XF, int YF, int ZF, int FORM, int BYT, int RY>
+template <int XF, int YF, int ZF, int FORM, int BYT, int RY, bool CURV>
 __global__ __launch_bounds__(2 * TX * BYT) __attribute__((amdgpu_waves_per_eu(4))) void k_stage(StageArgs g) {
   using L = StageLds<BYT, RY>;
   __shared__ float sh[L::SIZE];
@@ -973,18 +1105,18 @@ __global__ __launch_bounds__(2 * TX * BYT) __attribute__((amdgpu_waves_per_eu(4)
   float* const ss = sh + L::VOFF;
   if (wave < BYT) {
     if (it_ < g.p1)
-      vel_body<XF, YF, ZF, FORM, BYT, RY, TXN, true>(g, it, slot, sv, ss, wave);
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, sv, ss, wave);
     else if (it_ < g.p2)
-      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, true>(g, it, slot, sv, ss, wave);
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, true, CURV>(g, it, slot, sv, ss, wave);
     else
-      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, false>(g, it, slot, sv, ss, wave);
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, sv, ss, wave);
   } else {
     if (it_ < g.p1)
-      str_body<XF, YF, ZF, FORM, BYT, RY, TXN, true>(g, it, slot, ss, sv, wave - BYT);
+      str_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, ss, sv, wave - BYT);
     else if (it_ < g.p2)
-      str_body<XF, YF, ZF, FORM, BYT, RY, TX, true>(g, it, slot, ss, sv, wave - BYT);
+      str_body<XF, YF, ZF, FORM, BYT, RY, TX, true, CURV>(g, it, slot, ss, sv, wave - BYT);
     else
-      str_body<XF, YF, ZF, FORM, BYT, RY, TX, false>(g, it, slot, ss, sv, wave - BYT);
+      str_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, ss, sv, wave - BYT);
   }
 }
 
@@ -1007,5 +1139,14 @@ StageFn stage_fn_2_0(int pat, int form);
 StageFn stage_fn_2_1(int pat, int form);
 StageFn stage_fn_3_0(int pat, int form);
 StageFn stage_fn_3_1(int pat, int form);
+// mode C (terrain-following grid) instantiations of the same shapes
+StageFn stage_fnc_0_0(int pat, int form);
+StageFn stage_fnc_0_1(int pat, int form);
+StageFn stage_fnc_1_0(int pat, int form);
+StageFn stage_fnc_1_1(int pat, int form);
+StageFn stage_fnc_2_0(int pat, int form);
+StageFn stage_fnc_2_1(int pat, int form);
+StageFn stage_fnc_3_0(int pat, int form);
+StageFn stage_fnc_3_1(int pat, int form);
 
 }  // namespace fw

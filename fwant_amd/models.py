@@ -15,6 +15,9 @@ Presets (sizes and values of the named workloads):
                            with +-5% correlated perturbation, Vp = 1.73 Vs,
                            Gardner density, Ricker 1 Hz force
   small_test(...)          a reduced heterogeneous case for parity tests
+  hills_400x400x200()      gradient_400x400x200 on a terrain-following grid
+                           under 20 seeded Gaussian hills (mode C)
+  hills_test(...)          small_test under steep hills (mode C parity)
 """
 from dataclasses import dataclass, field
 from typing import Optional
@@ -62,6 +65,9 @@ class Model:
     moment: Sources = field(default_factory=lambda: Sources(comps=np.zeros((6, 0), np.float32)))
     rec_indx: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int32))  # 1-based
     rec_tinv: int = 1
+    # mode C: radius of every node of the ghosted grid, float64 (nx, ny, nz)
+    # Fortran order (a terrain-following shell); None: the spherical grid of z
+    r3: Optional[np.ndarray] = None
 
     @property
     def nx(self):
@@ -288,3 +294,92 @@ def curved_test(ni=40, nj=36, nk=24, nt=96, seed=41, colat0=1.0, r_top=20000.0, 
     dt = float(np.float32(0.32 * dmin / vmax))
     return small_test(ni=ni, nj=nj, nk=nk, nt=nt, h=h, seed=seed, colat0=colat0, r_top=r_top, dt=dt,
                       fscale=1000.0 / h * 0.8, **kw)
+
+
+# ---------------------------------------------------------------------------
+# Mode C: terrain-following grids (the reference's grid is a spherical shell;
+# these workloads add topography).  A node (i, j, k) sits at colatitude x(i),
+# longitude y(j) and radius
+#     r = z(k) + topo(i, j) w(k),
+# w rising smoothly (s^3 (10 - 15 s + 6 s^2), s the height fraction) from 0
+# at plane k_flat to 1 at the top interior plane nk2 and staying 1 above.
+# The stage kernels need the grid flat (p = q = 0, r = z) on every CFS-PML
+# slab cell: topography is tapered to zero over the x / y slabs and 3 more
+# columns, and w is 0 up to 3 planes above the bottom slab.
+# ---------------------------------------------------------------------------
+def terrain_radius(z, topo, k_flat, nk2):
+    """r(i, j, k) = z(k) + topo(i, j) w(k) (float64, Fortran order)."""
+    k = np.arange(len(z), dtype=np.float64)
+    s = np.clip((k - k_flat) / float(nk2 - k_flat), 0.0, 1.0)
+    w = s ** 3 * (10.0 - 15.0 * s + 6.0 * s * s)
+    r = z[None, None, :] + topo[:, :, None] * w[None, None, :]
+    return np.asfortranarray(r)
+
+
+def _edge_taper(n, flat, ramp):
+    """1-D window: 0 on `flat` entries at each end, a raised-cosine ramp of
+    `ramp` entries, 1 inside."""
+    t = np.ones(n)
+    i = np.arange(n, dtype=np.float64)
+    d = np.minimum(i - (flat - 1), (n - flat) - i)  # distance from the flat band
+    t = np.where(d <= 0, 0.0, np.where(d >= ramp, 1.0, 0.5 - 0.5 * np.cos(np.pi * d / ramp)))
+    return t
+
+
+def gaussian_hills(x, y, r_top, n, seed, amp=(0.0, 2000.0), sigma=(5000.0, 20000.0)):
+    """Sum of n Gaussian hills (numpy default_rng(seed): centres uniform over
+    the grid, amplitude and width uniform in the given ranges, metres) as a
+    height over the (x, y) nodes; distances are arcs at radius r_top."""
+    rng = np.random.default_rng(seed)
+    xc = rng.uniform(x[0], x[-1], n)
+    yc = rng.uniform(y[0], y[-1], n)
+    a = rng.uniform(amp[0], amp[1], n)
+    sg = rng.uniform(sigma[0], sigma[1], n)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    topo = np.zeros(X.shape)
+    for q in range(n):
+        d2 = ((X - xc[q]) * r_top) ** 2 + ((Y - yc[q]) * r_top * np.sin(xc[q])) ** 2
+        topo += a[q] * np.exp(-0.5 * d2 / (sg[q] * sg[q]))
+    return topo
+
+
+def with_topography(m, topo, ramp=8):
+    """Mode C copy of model m: the grid under the heights topo (nx, ny),
+    tapered to zero over the CFS-PML shell and 3 columns more, and w zero up
+    to 3 planes above the bottom slab.  The media stay as they are in index
+    space (layers follow the terrain).  The sources and receivers keep their
+    nodes."""
+    import copy
+    n = copy.copy(m)
+    nab = m.abs_number
+    # a slab of N points spans N + 1 nodes from the first (last) interior
+    # node; it and 3 nodes beyond stay flat
+    fx = LENFD + max(nab[0], nab[1]) + 4
+    fy = LENFD + max(nab[2], nab[3]) + 4
+    t = _edge_taper(m.nx, fx, ramp)[:, None] * _edge_taper(m.ny, fy, ramp)[None, :]
+    k_flat = LENFD + nab[4] + 3
+    n.r3 = terrain_radius(np.asarray(m.z, np.float64), topo * t, k_flat, m.nk2)
+    n.name = m.name + "+topo"
+    return n
+
+
+def hills_400x400x200(nt=3000, ni=400, nj=400, nk=200, h=500.0):
+    """gradient_400x400x200 under 20 Gaussian hills (numpy default_rng(2):
+    amplitude 0-2 km, width sigma 5-20 km), tapered to zero over the 12-point
+    CFS-PML shell (mode C)."""
+    m = gradient_400x400x200(nt=nt, ni=ni, nj=nj, nk=nk, h=h)
+    topo = gaussian_hills(m.x, m.y, R_EARTH, 20, seed=2)
+    n = with_topography(m, topo)
+    n.name = f"hills_{ni}x{nj}x{nk}"
+    return n
+
+
+def hills_test(ni=40, nj=36, nk=28, nt=96, seed=7, amp=3000.0, sigma=6000.0, nhill=3, **kw):
+    """small_test under nhill steep Gaussian hills (mode C parity): heights
+    up to amp, width sigma, slopes up to ~0.3 on the default 1 km grid."""
+    m = small_test(ni=ni, nj=nj, nk=nk, nt=nt, seed=seed, **kw)
+    topo = gaussian_hills(m.x, m.y, R_EARTH, nhill, seed=seed + 100, amp=(0.5 * amp, amp),
+                          sigma=(0.7 * sigma, sigma))
+    n = with_topography(m, topo, ramp=4)
+    n.name = "hills_test"
+    return n

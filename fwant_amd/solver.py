@@ -61,6 +61,15 @@ class Solver:
             check(L.fw_set_grid(h, fptr(xf), fptr(yf), fptr(zf), fptr(m["xsin"]), fptr(m["xcot"]),
                                 fptr(m["xi_x"]), fptr(m["eta_y"]), fptr(m["zeta_z"])), h, "fw_set_grid")
             self.metric = m
+        self.metric3d = None
+        if getattr(model, "r3", None) is not None:
+            # mode C: the terrain-following grid's 3-D metric, built on the GPU in FP64
+            r3 = np.asfortranarray(model.r3, np.float64).ravel(order="F")
+            shp = (self.nx, self.ny, self.nz)
+            m3 = {k: np.zeros(self.nx * self.ny * self.nz, np.float32) for k in ("p", "q", "zz", "rr")}
+            check(L.fw_build_metric3d(h, dptr(x), dptr(y), dptr(r3), fptr(m3["p"]), fptr(m3["q"]), fptr(m3["zz"]),
+                                      fptr(m3["rr"])), h, "fw_build_metric3d")
+            self.metric3d = {k: v.reshape(shp, order="F") for k, v in m3.items()}
         lam = np.asfortranarray(model.lam, np.float32).ravel(order="F")
         mu = np.asfortranarray(model.mu, np.float32).ravel(order="F")
         rho = np.asfortranarray(model.rho, np.float32).ravel(order="F")

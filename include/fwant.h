@@ -80,6 +80,25 @@ int fw_set_grid(fw_handle h, const float *x, const float *y, const float *z, con
  * copied back (lengths as in fw_set_grid). */
 int fw_build_metric(fw_handle h, const double *x, const double *y, const double *z, float *out_xsin,
                     float *out_xcot, float *out_xi_x, float *out_eta_y, float *out_zeta_z);
+/* Mode C: a terrain-following grid under topography.  No counterpart in the
+ * reference, whose grid is a spherical shell with the 1-D metric above (its
+ * coord files hold z(K) only); this is the 3-D extension of seis3d_metric
+ * (code/srcF/seis3d_metric.F90:82-112) for the "curvilinear" workloads.
+ * Node (i, j, k) sits at colatitude x[i], longitude y[j] and radius
+ * r[i + nx*(j + ny*k)] (ghosted, column-major, metres, FP64).  Builds on the
+ * GPU in FP64 the slopes p = -xi_x r_xi / r, q = -eta_y r_eta / (r sin x),
+ * zeta_r = 1 / r_zeta and 1/r (stored FP32, with the one-sided zeta
+ * derivatives of p and q), and switches the solver to the mode C scheme:
+ * derivatives d/dtheta = xi_x d_xi + zeta_theta d_zeta etc., traction
+ * imaging of sigma . (p, q, 1) at the free surface, source volumes and
+ * surface areas at the nodes' radii.  Call after fw_set_grid /
+ * fw_build_metric (which give xi_x, eta_y, xsin, xcot) and before fw_init.
+ * fw_init rejects a grid that is not flat (r = z(k)) on every CFS-PML slab
+ * cell and 3 cells beyond, and fluid cells.  A flat r (r = z(k)) runs the
+ * spherical scheme up to rounding.  out_* (nx*ny*nz each, dense) may be
+ * NULL; else p, q, zeta_r, 1/r are copied back. */
+int fw_build_metric3d(fw_handle h, const double *x, const double *y, const double *r, float *out_p,
+                      float *out_q, float *out_zeta_r, float *out_rr);
 /* Replaces media_import (code/srcF/mod_media.F90:241-255): ghosted nx*ny*nz arrays. */
 int fw_set_media(fw_handle h, const float *lambda, const float *mu, const float *rho);
 

@@ -130,6 +130,11 @@ typedef struct {
   int *pt_indx; /* 3*npt C indices */
   int nthreads;
   int cycle_origin; /* ntime at which the 8-step F/B cycle starts (a restarted run starts it anew) */
+  /* mode C (terrain-following shell, r = r(i, j, k)): 0 = the reference's
+   * spherical scheme; else the 3-D metric below (ncell each) */
+  int curv;
+  real *r3, *cp, *cq, *czz, *crr; /* radius, slopes p q, zeta_r, 1/r */
+  real *tau[3];                   /* traction combinations of one sweep */
 } orc_solver;
 
 #define IDX(s, i, j, k) ((long)(i) + (s)->sy * (long)(j) + (s)->sz * (long)(k))
@@ -144,7 +149,10 @@ typedef struct {
  *   6  smoothed body force also applied on the free-surface plane
  *   7  smoothed body force doubled
  *   8  surface renormalisation of the smoothing weights skipped
- *   9  surface traction of a surface force doubled */
+ *   9  surface traction of a surface force doubled
+ *  10  mode C: sign of the slope p flipped in the free-surface solve
+ *  11  mode C: the slope terms of the velocity update's zeta part left out
+ *  12  mode C: sign of the slope term p*aZ of E13 flipped */
 static int g_mut = 0;
 void ORC(set_mutation)(int m) { g_mut = m; }
 
@@ -334,6 +342,8 @@ void ORC(destroy)(orc_solver *s) {
   free(s->moment_indx); free(s->moment_shift); free(s->moment_t0); free(s->mom_time); free(s->mom_freq);
   free(s->Mxx); free(s->Myy); free(s->Mzz); free(s->Mxy); free(s->Mxz); free(s->Myz);
   free(s->pt_indx);
+  free(s->r3); free(s->cp); free(s->cq); free(s->czz); free(s->crr);
+  for (int q = 0; q < 3; q++) free(s->tau[q]);
   free(s);
 }
 
@@ -370,6 +380,75 @@ void ORC(build_metric)(int nx, int ny, int nz, const double *gx, const double *g
   metric_1d(gx, nx, LENFD, nx - 1 - LENFD, xi_x);
   metric_1d(gy, ny, LENFD, ny - 1 - LENFD, eta_y);
   metric_1d(gz, nz, LENFD, nz - 1 - LENFD, zeta_z);
+}
+
+/* ------------------------------------------------------------------------
+ * Mode C metric (no counterpart in the reference, whose grid is a spherical
+ * shell with a 1-D metric).  Terrain-following shell: theta = x(i),
+ * phi = y(j), r = r(i, j, k).  With xi_theta = xi_x(i), eta_phi = eta_y(j)
+ * (the 1-D metric above) the physical derivatives are
+ *   d/dtheta|_r = xi_x d_xi + zeta_theta d_zeta,   zeta_theta = -r_xi xi_x / r_zeta
+ *   d/dphi|_r   = eta_y d_eta + zeta_phi d_zeta,   zeta_phi   = -r_eta eta_y / r_zeta
+ *   d/dr        = zeta_r d_zeta,                   zeta_r     = 1 / r_zeta
+ * stored as the slopes p = zeta_theta / (r zeta_r) = -xi_x r_xi / r and
+ * q = zeta_phi / (r sin(theta) zeta_r) = -eta_y r_eta / (r sin(theta)), zeta_r
+ * and 1/r.  Derivatives of r use the metric operator of seis3d_metric
+ * (1/2 (F + B), 5 taps): along zeta in the expression of metric_1d, so that a
+ * flat grid gives zeta_r = zeta_z(k) exactly; along xi and eta with B
+ * written as the mirrored F taps, so that a flat grid gives p = q = 0
+ * exactly.  Derivative values at the 3 outer planes are copied from the
+ * nearest plane the stencil fits (the ghost rule of metric_1d); 1/r is
+ * evaluated everywhere.  FP64; 1/r from the FP32-rounded radius (as the
+ * spherical scheme takes 1/z of the FP32 z).
+ * Pinned by tests/test_oracle_topo.py (flat grid = the spherical metric,
+ * slopes of an analytic surface, the RHS on analytic fields).
+ * ---------------------------------------------------------------------- */
+static const double MC_F[5] = {-0.30874, -0.6326, 1.2330, -0.3334, 0.04168};
+static const double MC_B[5] = {-0.04168, 0.3334, -1.2330, 0.6326, 0.30874};
+static int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
+/* 1/2 (F + B) along a stride, B as mirrored F taps: exactly 0 on constants */
+static double dmet_mirror(const double *g, long st) {
+  double f = 0, b = 0;
+  for (int q = 0; q < 5; q++) { f += g[(q - 1) * st] * MC_F[q]; b -= g[(1 - q) * st] * MC_F[q]; }
+  return (f + b) * 0.5;
+}
+void ORC(build_metric3d)(int nx, int ny, int nz, const double *gx, const double *gy, const double *r3,
+                         double *p, double *q, double *zz, double *rr) {
+  const long sy = nx, sz = (long)nx * ny;
+  double *xi = (double *)malloc(sizeof(double) * nx), *et = (double *)malloc(sizeof(double) * ny);
+  metric_1d(gx, nx, LENFD, nx - 1 - LENFD, xi);
+  metric_1d(gy, ny, LENFD, ny - 1 - LENFD, et);
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const long c = i + sy * j + sz * k;
+        const int kc = clampi(k, LENFD, nz - 1 - LENFD), ic = clampi(i, LENFD, nx - 1 - LENFD),
+                  jc = clampi(j, LENFD, ny - 1 - LENFD);
+        /* zeta_r: metric_1d's expression on the column (i, j) */
+        const double *g = r3 + i + sy * j;
+        double b = 0, f = 0;
+        for (int m = 0; m < 5; m++) { b += g[(kc - 3 + m) * sz] * MC_B[m]; f += g[(kc - 1 + m) * sz] * MC_F[m]; }
+        zz[c] = 1.0 / ((b + f) * 0.5);
+        const long cx = ic + sy * j + sz * k, cy = i + sy * jc + sz * k;
+        p[c] = -xi[ic] * dmet_mirror(r3 + cx, 1) / r3[cx];
+        q[c] = -et[jc] * dmet_mirror(r3 + cy, sy) / (r3[cy] * sin(gx[i]));
+        rr[c] = 1.0 / (double)(float)r3[c];
+      }
+  free(xi);
+  free(et);
+}
+/* switch the solver to mode C with the 3-D metric (ncell each: radius,
+ * slopes, zeta_r, 1/r) */
+void ORC(set_metric3d)(orc_solver *s, const real *r3, const real *p, const real *q, const real *zz, const real *rr) {
+  real **a[5] = {&s->r3, &s->cp, &s->cq, &s->czz, &s->crr};
+  const real *src[5] = {r3, p, q, zz, rr};
+  for (int n = 0; n < 5; n++) {
+    if (!*a[n]) *a[n] = zalloc(s->ncell);
+    memcpy(*a[n], src[n], sizeof(real) * s->ncell);
+  }
+  for (int n = 0; n < 3; n++)
+    if (!s->tau[n]) s->tau[n] = zalloc(s->ncell);
+  s->curv = 1;
 }
 
 void ORC(set_media)(orc_solver *s, const real *lam, const real *mu, const real *rho) {
@@ -569,8 +648,42 @@ static void abs_rk(orc_solver *s, int kind, real a, real b) {
 /* ------------------------------------------------------------------------
  * a4: free_timg, code/srcF/mod_macdrp.F90:3414-3436.
  * ---------------------------------------------------------------------- */
+/* Mode C traction imaging (no counterpart in the reference): the traction
+ * combinations across the surface zeta = const, scaled so that the radial
+ * stress comes first,
+ *   tau_x = Txz + p Txx + q Txy,  tau_y = Tyz + p Txy + q Tyy,  tau_z = Tzz + p Txz + q Tyz
+ * (sigma . N with N = (p, q, 1)), are set to the source traction at k = nk2
+ * by solving for Txz, Tyz, then Tzz, and imaged oddly above it: the planes
+ * nk2 + m hold tau(nk2 + m) = 2 tau_src - tau(nk2 - m) in Txz Tyz Tzz.  With
+ * p = q = 0 this is free_timg. */
+static void free_timg_c(orc_solver *s) {
+  real **W = s->W;
+  int nk2 = s->nk2;
+  for (int m = 1; m <= 3; m++)
+    for (int j = s->nj1; j <= s->nj2; j++)
+      for (int i = s->ni1; i <= s->ni2; i++) {
+        long c2 = i + s->sy * j, b = IDX(s, i, j, nk2 - m), a = IDX(s, i, j, nk2 + m);
+        real p = s->cp[b], q = s->cq[b];
+        real tx = W[TXZ][b] + p * W[TXX][b] + q * W[TXY][b];
+        real ty = W[TYZ][b] + p * W[TXY][b] + q * W[TYY][b];
+        real tz = W[TZZ][b] + p * W[TXZ][b] + q * W[TYZ][b];
+        W[TXZ][a] = C_(2.0) * s->TxSrc[c2] - tx;
+        W[TYZ][a] = C_(2.0) * s->TySrc[c2] - ty;
+        W[TZZ][a] = C_(2.0) * s->TzSrc[c2] - tz;
+      }
+  for (int j = s->nj1; j <= s->nj2; j++)
+    for (int i = s->ni1; i <= s->ni2; i++) {
+      long c2 = i + s->sy * j, c = IDX(s, i, j, nk2);
+      real p = s->cp[c], q = s->cq[c];
+      W[TXZ][c] = s->TxSrc[c2] - p * W[TXX][c] - q * W[TXY][c];
+      W[TYZ][c] = s->TySrc[c2] - p * W[TXY][c] - q * W[TYY][c];
+      W[TZZ][c] = s->TzSrc[c2] - p * W[TXZ][c] - q * W[TYZ][c];
+    }
+}
+
 static void free_timg(orc_solver *s) {
   if (!s->freenode) return;
+  if (s->curv) { free_timg_c(s); return; }
   int nk2 = s->nk2;
   for (int k = 1; k <= 3; k++)
     for (int j = s->nj1; j <= s->nj2; j++)
@@ -653,11 +766,149 @@ static void op_sweep(orc_solver *s, int xf, int yf, int zf) {
       }
 }
 
+/* ------------------------------------------------------------------------
+ * Mode C operator sweep (no counterpart in the reference).  The spherical
+ * velocity-stress equations of op_sweep with the physical derivatives of a
+ * terrain-following grid: d/dtheta = xi_x F_xi + zeta_theta F_zeta etc.
+ * (build_metric3d).  Velocity half: the zeta parts of a row of the stress
+ * divergence, zeta_r (F(sigma_ir) + p F(sigma_itheta) + q F(sigma_iphi)),
+ * are evaluated as zeta_r (F(tau_i) - sigma_itheta F(p) - sigma_iphi F(q))
+ * with the traction combinations tau of free_timg_c, so that the odd images
+ * above the free surface enter only through tau.  Stress half: with
+ * a_i = zeta_r F(V_i) (F of the VLOW orders next to the surface),
+ *   E11 = (Dx Vx + Vz)/r + p aX,  E22 = (Vx cot + Dy Vy / sin + Vz)/r + q aY,
+ *   E12 = ((Dx Vy + Dy Vx / sin - Vy cot)/r + p aY + q aX)/2,  E33 = aZ,
+ *   E13 = (Dx Vz / r + aX - Vx / r + p aZ)/2,  E23 = (Dy Vz / (r sin) + aY - Vy / r + q aZ)/2
+ * (Dx, Dy: the xi and eta parts).  At the free surface, a is solved from the
+ * traction-rate condition (sigma_t . N = S, surf_solve_c); where p = q = 0
+ * the reference's traction condition is used as written.  With p = q = 0
+ * and r = z(k) everywhere the sweep is op_sweep up to rounding.
+ * Pinned by tests/test_oracle_topo.py: the flat grid against op_sweep, the
+ * RHS on analytic fields against the continuous equations on a curved
+ * grid, the surface solve's traction rates, the Rayleigh speed along a
+ * sloping surface.
+ * ---------------------------------------------------------------------- */
+/* Free-surface strain rates on a sloping surface: E holds the strain rates
+ * of the xi / eta derivatives (E11 E22 E33 E12 E13 E23 with E33 = 0 and the
+ * zeta parts left out); a = zeta_r F(V) is the solution of
+ *   (mu n2 I + (lam + mu) u u^T) a = S - sigma(E) . N,  u = N = (p, q, 1), n2 = u.u
+ * (the traction rate of Hooke's law with the zeta parts, by Sherman-Morrison),
+ * and E receives the zeta parts. */
+static void surf_solve_c(real lam, real miu, real p, real q, real Sx, real Sy, real Sz, real *E) {
+  real lam2mu = lam + C_(2.0) * miu;
+  if (g_mut == 10) p = -p;
+  real tr = E[0] + E[1];
+  real sxx = lam * tr + C_(2.0) * miu * E[0], syy = lam * tr + C_(2.0) * miu * E[1], szz = lam * tr;
+  real sxy = C_(2.0) * miu * E[3], sxz = C_(2.0) * miu * E[4], syz = C_(2.0) * miu * E[5];
+  real R0 = Sx - (p * sxx + q * sxy + sxz);
+  real R1 = Sy - (p * sxy + q * syy + syz);
+  real R2 = Sz - (p * sxz + q * syz + szz);
+  real n2 = C_(1.0) + p * p + q * q;
+  real g = (lam + miu) / (lam2mu * n2) * (p * R0 + q * R1 + R2);
+  real mn = miu * n2;
+  real aX = (R0 - g * p) / mn, aY = (R1 - g * q) / mn, aZ = (R2 - g) / mn;
+  E[0] = E[0] + p * aX;
+  E[1] = E[1] + q * aY;
+  E[2] = aZ;
+  E[3] = E[3] + (p * aY + q * aX) * C_(0.5);
+  E[4] = E[4] + (aX + p * aZ) * C_(0.5);
+  E[5] = E[5] + (aY + q * aZ) * C_(0.5);
+}
+static void op_sweep_c(orc_solver *s, int xf, int yf, int zf) {
+  real **W = s->W, **h = s->h, **tau = s->tau;
+  const long sy = s->sy, sz = s->sz;
+  const int nk2 = s->nk2, freenode = s->freenode;
+  /* traction combinations on every plane of the computed columns */
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int k = 0; k < s->nz; k++)
+    for (int j = s->nj1; j <= s->nj2; j++)
+      for (int i = s->ni1; i <= s->ni2; i++) {
+        long c = IDX(s, i, j, k);
+        if (freenode && k > nk2) {
+          tau[0][c] = W[TXZ][c]; tau[1][c] = W[TYZ][c]; tau[2][c] = W[TZZ][c];
+        } else {
+          real p = s->cp[c], q = s->cq[c];
+          if (g_mut == 11) { p = 0; q = 0; }
+          tau[0][c] = W[TXZ][c] + p * W[TXX][c] + q * W[TXY][c];
+          tau[1][c] = W[TYZ][c] + p * W[TXY][c] + q * W[TYY][c];
+          tau[2][c] = W[TZZ][c] + p * W[TXZ][c] + q * W[TYZ][c];
+        }
+      }
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+  for (int k = s->nk1; k <= nk2; k++)
+    for (int j = s->nj1; j <= s->nj2; j++)
+      for (int i = s->ni1; i <= s->ni2; i++) {
+        long c = IDX(s, i, j, k);
+        real lam = s->lam[c], miu = s->mu[c], lam2mu = lam + C_(2.0) * miu, rrho = C_(1.0) / s->rho[c];
+        real xi = s->xi_x[i], et = s->eta_y[j], ze = s->czz[c], rzk = s->crr[c];
+        real p = s->cp[c], q = s->cq[c];
+        real rxs = C_(1.0) / s->xsin[i], xc = s->xcot[i];
+        real DxTxx = opD(xf, W[TXX] + c, 1) * xi, DxTxy = opD(xf, W[TXY] + c, 1) * xi;
+        real DxTxz = opD(xf, W[TXZ] + c, 1) * xi, DxVx = opD(xf, W[VX] + c, 1) * xi;
+        real DxVy = opD(xf, W[VY] + c, 1) * xi, DxVz = opD(xf, W[VZ] + c, 1) * xi;
+        real DyTyy = opD(yf, W[TYY] + c, sy) * et, DyTxy = opD(yf, W[TXY] + c, sy) * et;
+        real DyTyz = opD(yf, W[TYZ] + c, sy) * et, DyVx = opD(yf, W[VX] + c, sy) * et;
+        real DyVy = opD(yf, W[VY] + c, sy) * et, DyVz = opD(yf, W[VZ] + c, sy) * et;
+        real Txx = W[TXX][c], Tyy = W[TYY][c], Tzz = W[TZZ][c], Txy = W[TXY][c], Txz = W[TXZ][c],
+             Tyz = W[TYZ][c], Vx = W[VX][c], Vy = W[VY][c], Vz = W[VZ][c];
+        real dp = opD(zf, s->cp + c, sz), dq = opD(zf, s->cq + c, sz);
+        if (g_mut == 11) { dp = 0; dq = 0; }
+        real Zx = (opD(zf, tau[0] + c, sz) - Txx * dp - Txy * dq) * ze;
+        real Zy = (opD(zf, tau[1] + c, sz) - Txy * dp - Tyy * dq) * ze;
+        real Zz = (opD(zf, tau[2] + c, sz) - Txz * dp - Tyz * dq) * ze;
+        real aX, aY, aZ;
+        if (freenode && k == nk2 - 1) {
+          aX = op22(zf, W[VX] + c, sz) * ze; aY = op22(zf, W[VY] + c, sz) * ze; aZ = op22(zf, W[VZ] + c, sz) * ze;
+        } else if (freenode && k == nk2 - 2) {
+          aX = op24(zf, W[VX] + c, sz) * ze; aY = op24(zf, W[VY] + c, sz) * ze; aZ = op24(zf, W[VZ] + c, sz) * ze;
+        } else {
+          aX = opD(zf, W[VX] + c, sz) * ze; aY = opD(zf, W[VY] + c, sz) * ze; aZ = opD(zf, W[VZ] + c, sz) * ze;
+        }
+        h[VX][c] = rrho * (DxTxx * rzk + DyTxy * rzk * rxs + Zx + (C_(3.0) * Txz + Txx * xc - Tyy * xc) * rzk);
+        h[VY][c] = rrho * (DxTxy * rzk + DyTyy * rzk * rxs + Zy + (C_(2.0) * Txy * xc + C_(3.0) * Tyz) * rzk);
+        h[VZ][c] = rrho * (DxTxz * rzk + DyTyz * rzk * rxs + Zz + (C_(2.0) * Tzz - Txx - Tyy + Txz * xc) * rzk);
+        real E[6];
+        if (freenode && k == nk2) {
+          long c2 = i + sy * j;
+          if (p == 0 && q == 0) {
+            E[0] = (DxVx + Vz) * rzk;
+            E[1] = (Vx * xc + DyVy * rxs + Vz) * rzk;
+            E[3] = (DxVy + DyVx * rxs - Vy * xc) * rzk * C_(0.5);
+            E[2] = -lam / lam2mu * (E[0] + E[1]) + s->VzSrc[c2] / lam2mu;
+            E[4] = s->VxSrc[c2] / miu / C_(2.0);
+            E[5] = s->VySrc[c2] / miu / C_(2.0);
+          } else {
+            E[0] = (DxVx + Vz) * rzk;
+            E[1] = (Vx * xc + DyVy * rxs + Vz) * rzk;
+            E[2] = 0;
+            E[3] = (DxVy + DyVx * rxs - Vy * xc) * rzk * C_(0.5);
+            E[4] = (DxVz * rzk - Vx * rzk) * C_(0.5);
+            E[5] = (DyVz * rzk * rxs - Vy * rzk) * C_(0.5);
+            surf_solve_c(lam, miu, p, q, s->VxSrc[c2], s->VySrc[c2], s->VzSrc[c2], E);
+          }
+        } else {
+          E[0] = (DxVx + Vz) * rzk + p * aX;
+          E[1] = (Vx * xc + DyVy * rxs + Vz) * rzk + q * aY;
+          E[3] = ((DxVy + DyVx * rxs - Vy * xc) * rzk + p * aY + q * aX) * C_(0.5);
+          E[2] = aZ;
+          E[4] = (DxVz * rzk + aX - Vx * rzk + (g_mut == 12 ? -p : p) * aZ) * C_(0.5);
+          E[5] = (DyVz * rzk * rxs + aY - Vy * rzk + q * aZ) * C_(0.5);
+        }
+        h[TXX][c] = lam2mu * E[0] + lam * (E[1] + E[2]);
+        h[TYY][c] = lam2mu * E[1] + lam * (E[0] + E[2]);
+        h[TZZ][c] = lam2mu * E[2] + lam * (E[0] + E[1]);
+        h[TXY][c] = C_(2.0) * miu * E[3];
+        h[TXZ][c] = C_(2.0) * miu * E[4];
+        h[TYZ][c] = C_(2.0) * miu * E[5];
+      }
+}
+
 /* exported: evaluate the RHS of one operator pattern on the current state
  * (after the free-surface imaging a stage would apply) into h (9 fields) */
 void ORC(rhs)(orc_solver *s, int xf, int yf, int zf, real *hout /* 9*ncell */) {
   free_timg(s);
-  op_sweep(s, xf, yf, zf);
+  if (s->curv) op_sweep_c(s, xf, yf, zf);
+  else op_sweep(s, xf, yf, zf);
   for (int f = 0; f < NFIELD; f++) memcpy(hout + (long)f * s->ncell, s->h[f], sizeof(real) * s->ncell);
 }
 
@@ -806,6 +1057,13 @@ static void abs_correct(orc_solver *s, int xf, int yf, int zf) {
  * Stokes' solutions (tests/test_oracle_analytic.py).
  * ---------------------------------------------------------------------- */
 static real cell_volume(orc_solver *s, int i, int j, int k) {
+  if (s->curv) {
+    /* mode C: the radial extent along the column, the radius of the cell */
+    long c = IDX(s, i, j, k);
+    real r = s->r3[c];
+    return (s->x[i + 1] - s->x[i - 1]) * (s->y[j + 1] - s->y[j - 1]) * (s->r3[c + s->sz] - s->r3[c - s->sz]) /
+           C_(8.0) * (r * r) * s->xsin[i];
+  }
   return (s->x[i + 1] - s->x[i - 1]) * (s->y[j + 1] - s->y[j - 1]) * (s->z[k + 1] - s->z[k - 1]) / C_(8.0) *
          (s->z[k] * s->z[k]) * s->xsin[i];
 }
@@ -829,7 +1087,10 @@ static void src_surface(orc_solver *s, int ntime, real tinc) {
       long q = m + (long)s->ntw_force * n;
       real fx0 = s->Fx[q], fy0 = s->Fy[q], fz0 = s->Fz[q];
       if (i <= s->ni2 && i >= s->ni1 && j <= s->nj2 && j >= s->nj1 && k == s->nk2) {
-        real S = (s->x[i + 1] - s->x[i - 1]) * (s->y[j + 1] - s->y[j - 1]) / C_(4.0) * (s->z[k] * s->z[k]) * s->xsin[i];
+        /* mode C: the projected area at the surface radius of the column
+         * (sigma . N = F / S_projected: N's length cancels the slope's area) */
+        real zs = s->curv ? s->r3[IDX(s, i, j, k)] : s->z[k];
+        real S = (s->x[i + 1] - s->x[i - 1]) * (s->y[j + 1] - s->y[j - 1]) / C_(4.0) * (zs * zs) * s->xsin[i];
         long c2 = i + s->sy * j;
         if (g_mut == 9) S = S / C_(2.0);
         s->TxSrc[c2] = fx0 / S * disp; s->TySrc[c2] = fy0 / S * disp; s->TzSrc[c2] = fz0 / S * disp;
@@ -937,7 +1198,8 @@ void ORC(step)(orc_solver *s, int ntime) {
     ORC(stage_pattern)(ntime - s->cycle_origin, st, pat);
     src_surface(s, ntime, tinc);
     free_timg(s);
-    op_sweep(s, pat[0], pat[1], pat[2]);
+    if (s->curv) op_sweep_c(s, pat[0], pat[1], pat[2]);
+    else op_sweep(s, pat[0], pat[1], pat[2]);
     abs_vsymm(s);
     abs_correct(s, pat[0], pat[1], pat[2]);
     src_stress(s, ntime, tinc);

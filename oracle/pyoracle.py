@@ -71,6 +71,8 @@ def _load(prec):
         "fp16_to_f32": (C.c_float, [C.c_uint16]),
         "set_mutation": (None, [C.c_int]),
         "src_terms": (None, [vp, C.c_int, R, rp, rp]),
+        "build_metric3d": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp, dp, dp]),
+        "set_metric3d": (None, [vp, rp, rp, rp, rp, rp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, p + name)
@@ -159,10 +161,24 @@ def build_metric(model):
     return dict(zip(("xsin", "xcot", "xi_x", "eta_y", "zeta_z"), out))
 
 
+def build_metric3d(model):
+    """Oracle FP64 mode C metric of a terrain-following model (model.r3):
+    slopes p, q, zeta_r and 1/r, each float64 (nx, ny, nz)."""
+    L = Lib(64).L
+    nx, ny, nz = model.nx, model.ny, model.nz
+    d = C.POINTER(C.c_double)
+    x, y = (np.ascontiguousarray(a, np.float64) for a in (model.x, model.y))
+    r3 = np.asfortranarray(model.r3, np.float64)
+    out = [np.zeros((nx, ny, nz), np.float64, order="F") for _ in range(4)]
+    L.orc64_build_metric3d(nx, ny, nz, x.ctypes.data_as(d), y.ctypes.data_as(d), r3.ctypes.data_as(d),
+                           *[o.ctypes.data_as(d) for o in out])
+    return dict(zip(("p", "q", "zz", "rr"), out))
+
+
 class OracleSolver:
     """The oracle solver on a Model (same inputs as fwant_amd.solver.Solver)."""
 
-    def __init__(self, model, prec=32, metric=None, nthreads=None):
+    def __init__(self, model, prec=32, metric=None, nthreads=None, metric3d=None):
         self.lib = lib = Lib(prec)
         dt = lib.dtype
         self.model = m = model
@@ -177,6 +193,13 @@ class OracleSolver:
         lib.set_grid(self.h, *[lib.ptr(a) for a in g])
         media = [np.asfortranarray(a, dt).ravel(order="F") for a in (m.lam, m.mu, m.rho)]
         lib.set_media(self.h, *[lib.ptr(a) for a in media])
+        if getattr(m, "r3", None) is not None:
+            # mode C: the oracle's own FP64 metric, or a given one (p, q, zz, rr)
+            met3 = build_metric3d(m) if metric3d is None else metric3d
+            arrs = [np.asfortranarray(a, dt).ravel(order="F") for a in
+                    (m.r3, met3["p"], met3["q"], met3["zz"], met3["rr"])]
+            self._keep += arrs
+            lib.set_metric3d(self.h, *[lib.ptr(a) for a in arrs])
         an = np.array(m.abs_number, np.int32)
         av, bm, am = (np.array(v, dt) for v in (m.abs_velocity, m.cfs_bmax, m.cfs_amax))
         lib.set_pml(self.h, an.ctypes.data_as(C.POINTER(C.c_int)), lib.ptr(av), lib.ptr(bm), lib.ptr(am))
