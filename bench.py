@@ -1,31 +1,39 @@
 #!/usr/bin/env python3
 """Throughput of the plain FP32 solver on one MI355X.
 
-Workload: gradient_400x400x200 -- 400x400x200 cells, h = 0.5 km, flat free
-surface on the spherical reference scheme, Vs gradient with a correlated
-+-5% perturbation, 12-point CFS-PML on 5 sides, Ricker 1 Hz surface force.
+Workload (default): hills_400x400x200 -- the curvilinear case: 400x400x200
+cells, h = 0.5 km, a terrain-following grid under 20 seeded Gaussian hills
+(0-2 km high, sigma 5-20 km, tapered to zero over the CFS-PML shell), Vs
+gradient 3.2 -> 4.6 km/s with a correlated +-5% perturbation, Vp = 1.73 Vs,
+Gardner density, 12-point CFS-PML on 5 sides, Ricker 1 Hz surface force
+(mode C: the 3-D metric).  --workload flat runs the same model on the
+spherical grid without hills (mode S, the reference's scheme); the default
+run also times mode S and reports it under "flat_spherical".
 A step is one full time step (4 RK stages) over all cells; inputs are
 resident in HBM before the timed region starts.
 
 Prints ONE JSON line:
   value      cell-updates/s (interior cells x steps / wall time of K steps)
   roofline   one RK stage (k_stage over all cells, the dominant work):
-             algorithmic bytes per stage (156 B per cell, the average of
-             120/192/192/120 B over the 4 RK stages of the 3-register RK4
-             form) over its mean duration measured with HIP events on the
-             solver's stream,
-             against the 8 TB/s HBM peak; traffic = HBM bytes per stage
-             from rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by the same
-             counters on a plain copy (summarised by --summarize-pmc into
-             profiles/stage_traffic_gradient_400x400x200.json)
+             algorithmic bytes per stage over its mean duration measured
+             with HIP events on the solver's stream, against the 8 TB/s HBM
+             peak.  Algorithmic bytes per cell and stage: mode S 156 B (the
+             average of 120/192/192/120 B over the 4 RK stages of the
+             3-register RK4 form: W, media, m, t); mode C 180 B (the same
+             plus the 6 metric values of a cell that the stage reads:
+             slopes p, q, zeta_r, 1/r and the zeta derivatives of p, q).
+             traffic = HBM bytes per stage from rocprofv3 FETCH_SIZE /
+             WRITE_SIZE runs, scaled by the same counters on a plain copy
+             (summarised by --summarize-pmc into
+             profiles/stage_traffic_<workload>.json)
   cpu_baseline  the FP32 C oracle (OpenMP) on every host core this
              process may run on, timed on a bounded sample of the same
              workload (2 full steps); nproc and the CPU model reported
   accuracy   receiver records of the two-layer 100x100x50 workload (1008
              steps) against the committed FP32 / FP64 CPU records
   pipeline   station-pair kernels/hour, plain (one FP32 solve per station,
-             SGT through host memory) vs batched + BF16 SGT (one batched
-             solve with step graphs and the autotuner, SGT in HBM), on the
+             SGT through host memory) vs batched + FP16 SGT (one batched
+             solve with step graphs and the autotuner, FP16 SGT in HBM), on the
              batched configuration at a tenth of its record lengths: 256 x
              256 x 128, 32 virtual sources of 150-400 steps, SGT every 10
              steps on the 2x decimated grid, 16 pairs; with the kernels'
@@ -41,9 +49,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_UPDATE = 624.0           # 156 FP32 values per cell per time step
-BYTES_PER_CELL_LAUNCH = BYTES_PER_CELL_UPDATE / 4
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "stage_traffic_gradient_400x400x200.json")
+# algorithmic FP32 bytes per cell-update (4 RK stages): mode S 156 values,
+# mode C 156 + 4 x 6 metric values
+BYTES_PER_CELL_UPDATE = {"flat": 624.0, "hills": 720.0}
+WORKLOAD_NAME = {"flat": "gradient_400x400x200", "hills": "hills_400x400x200"}
+
+
+def traffic_json(workload):
+    return os.path.join(ROOT, "profiles", f"stage_traffic_{WORKLOAD_NAME[workload]}.json")
 COPY_BYTES = 1 << 30                    # --copy-calibration moves 1 GiB each way per copy
 
 
@@ -86,7 +99,7 @@ def summarize_pmc(fetch_db, write_db, copy_fetch_db, copy_write_db, out):
     print(json.dumps(res))
 
 
-def summarize_trace(db, ncell):
+def summarize_trace(db, ncell, bytes_per_launch_cell):
     """Markdown table of kernel durations by family (and by RK form for the stage kernel)."""
     import re
     import sqlite3
@@ -108,7 +121,7 @@ def summarize_trace(db, ncell):
     if nst:
         mean = sum(st) / nst
         print(f"\nmean stage (k_stage) {mean:.1f} us -> "
-              f"{ncell * BYTES_PER_CELL_LAUNCH / (mean * 1e-6) / 1e9:.0f} GB/s algorithmic")
+              f"{ncell * bytes_per_launch_cell / (mean * 1e-6) / 1e9:.0f} GB/s algorithmic")
 
 
 def cpu_model():
@@ -129,17 +142,19 @@ def log(*a):
 
 def pipeline_sample():
     """Station-pair kernels/hour on the batched-sources configuration at a
-    tenth of its record lengths: 256x256x128 gradient model, 32 seeded
+    tenth of its record lengths: 256x256x128 curvilinear model (the gradient
+    model under 20 seeded hills, mode C), 32 seeded
     surface stations with record lengths uniform in 150-400 steps (the
     configuration's 1500-4000 over 10), 6-component stress SGT every 10
     steps on the 2x decimated grid (128x128x64), 16 seeded station pairs.
     plain: one FP32 solve per station whose SGT goes to host memory, each
     pair's two SGTs back to the GPU for the integral; batched: all 32 in one
-    autotuned, graph-replayed batched solve with BF16 SGT kept in HBM."""
+    autotuned, graph-replayed batched solve with FP16 SGT (per-block
+    scales) kept in HBM."""
     import numpy as np
     from scipy import signal
     from fwant_amd import models, pipeline
-    m = models.gradient_400x400x200(nt=400, ni=256, nj=256, nk=128, h=500.0, seed=4)
+    m = models.hills_256x256x128(nt=400)
     rng = np.random.default_rng(11)
     w = 256 - 48
     cells = rng.choice(w * w, size=32, replace=False)
@@ -158,23 +173,23 @@ def pipeline_sample():
     b, a = signal.butter(2, 0.4)
     kw = dict(tinv=tinv, step=2, V=V, U=U, win=(2, 14), M0=1e16, filt=("P2", b, a))
     kp, tp = pipeline.plain(m, st, nsteps, pairs, host_sgt=True, **kw)
-    kb, tb = pipeline.batched(m, st, nsteps, pairs, fmt="bf16", **kw)
+    kb, tb = pipeline.batched(m, st, nsteps, pairs, fmt="fp16", **kw)
     errs = [float(np.linalg.norm(y.astype(np.float64) - x) / np.linalg.norm(x))
             for P, B in zip(kp, kb) for x, y in zip(P, B) if np.linalg.norm(x) > 0]
     tot_p = sum(v for k, v in tp.items())
     tot_b = tb["setup"] + tb["steps"] + tb["kernel"]
     plain_h, batch_h = len(pairs) / tot_p * 3600.0, len(pairs) / tot_b * 3600.0
-    log(f"pipeline: plain {tp}, batched bf16 {tb}, kernel rel-L2 max {max(errs):.2e} over {len(errs)} kernels")
-    return {"workload": "256x256x128 gradient model, 32 virtual sources (record lengths 150-400 steps: the "
+    log(f"pipeline: plain {tp}, batched fp16 {tb}, kernel rel-L2 max {max(errs):.2e} over {len(errs)} kernels")
+    return {"workload": "hills_256x256x128 curvilinear model (mode C), 32 virtual sources (record lengths 150-400 steps: the "
                         "batched configuration's 1500-4000 over 10), 6-component stress SGT every 10 steps on the "
                         "2x decimated grid (128x128x64), 16 station pairs",
             "unit": "kernels/hour",
             "plain": dict(value=plain_h, sgt="FP32 via host memory", **tp),
-            "batched_bf16": dict(value=batch_h, sgt="BF16 codes in HBM", **tb),
+            "batched_fp16": dict(value=batch_h, sgt="FP16 codes with per-16^3-block FP32 scales in HBM", **tb),
             "speedup": batch_h / plain_h,
             "speedup_steps_only": tp["steps"] / tb["steps"],
-            "kernel_rel_l2_bf16_vs_fp32_gpu": max(errs), "kernels_compared": len(errs),
-            "kernel_error_vs_fp64_oracle": "tests/test_gpu_configs.py (16 pairs, BF16 and FP16)"}
+            "kernel_rel_l2_fp16_vs_fp32_gpu": max(errs), "kernels_compared": len(errs),
+            "kernel_error_vs_fp64_oracle": "tests/test_gpu_configs.py (16 pairs: FP16 2.1e-3, BF16 1.6e-2)"}
 
 
 GOLDEN = os.path.join(ROOT, "tests", "golden", "two_layer_100x100x50.npz")
@@ -205,11 +220,41 @@ def accuracy_sample():
             "rel_l2_vs_fp32_cpu": e32, "rel_l2_vs_fp64_cpu": e64, "fp32_cpu_vs_fp64_cpu": o64}
 
 
+def time_solver(m, steps, warmup, bytes_per_cell_update):
+    """(cell-updates/s, mean stage us, achieved GB/s, stage share of the
+    step) of the plain solver on model m: warmup untimed steps, then `steps`
+    timed between device synchronisations, then 32 steps with HIP events
+    around every stage launch on the solver's stream."""
+    from fwant_amd.solver import Solver
+    from fwant_amd._lib import device_sync
+    s = Solver(m, max_records=steps + warmup + 64)
+    ncell = m.ni * m.nj * m.nk
+    s.run(warmup, ntime0=0)
+    s.sync()
+    device_sync()
+    t0 = time.perf_counter()
+    s.run(steps, ntime0=warmup)
+    s.sync()
+    device_sync()
+    el = time.perf_counter() - t0
+    value = ncell * steps / el
+    s.timing(True)
+    s.run(min(steps, 32), ntime0=warmup + steps)
+    stage_ms, total_ms, nst = s.get_timing()
+    s.timing(False)
+    s.close()
+    avg_ms = stage_ms / max(nst, 1)
+    achieved = ncell * bytes_per_cell_update / 4 / (avg_ms * 1e-3) / 1e9
+    return value, el, avg_ms * 1e3, achieved, stage_ms / total_ms
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=16)
+    ap.add_argument("--workload", choices=("hills", "flat"), default="hills",
+                    help="hills: the curvilinear 400x400x200 case (mode C); flat: the same without hills (mode S)")
     ap.add_argument("--ni", type=int, default=400)
     ap.add_argument("--nj", type=int, default=400)
     ap.add_argument("--nk", type=int, default=200)
@@ -217,7 +262,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--traffic", type=str, default=TRAFFIC_JSON,
+    ap.add_argument("--no-flat", action="store_true", help="do not also time the flat (mode S) variant")
+    ap.add_argument("--traffic", type=str, default=None,
                     help="JSON file with measured HBM bytes per RK stage (written by --summarize-pmc)")
     ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_DB", "WRITE_DB", "COPY_FETCH_DB", "COPY_WRITE_DB", "OUT"),
                     help="write the traffic JSON from rocprofv3 --pmc results databases and exit")
@@ -232,7 +278,7 @@ def main():
         summarize_pmc(*args.summarize_pmc)
         return
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace, args.ni * args.nj * args.nk)
+        summarize_trace(args.summarize_trace, args.ni * args.nj * args.nk, BYTES_PER_CELL_UPDATE[args.workload] / 4)
         return
     if args.copy_calibration:
         import numpy as np
@@ -253,44 +299,36 @@ def main():
         sys.exit(2)
 
     from fwant_amd import models
-    from fwant_amd.solver import Solver
 
+    wl = args.workload
+    bpc = BYTES_PER_CELL_UPDATE[wl]
     t = time.time()
-    m = models.gradient_400x400x200(ni=args.ni, nj=args.nj, nk=args.nk)
+    kw = dict(ni=args.ni, nj=args.nj, nk=args.nk)
+    m = models.hills_400x400x200(**kw) if wl == "hills" else models.gradient_400x400x200(**kw)
     log(f"model {m.name} {m.ni}x{m.nj}x{m.nk} built in {time.time() - t:.1f}s")
-    s = Solver(m, max_records=args.steps + args.warmup + 64)
     ncell = m.ni * m.nj * m.nk
-    from fwant_amd._lib import device_sync
-
-    s.run(args.warmup, ntime0=0)
-    s.sync()
-    device_sync()
-    t0 = time.perf_counter()
-    s.run(args.steps, ntime0=args.warmup)
-    s.sync()
-    device_sync()
-    el = time.perf_counter() - t0
-    value = ncell * args.steps / el
+    value, el, mean_us, achieved, share = time_solver(m, args.steps, args.warmup, bpc)
     log(f"timed {args.steps} steps in {el:.3f}s -> {value / 1e9:.3f} G cell-updates/s")
-
-    # live per-launch timing of the stage kernel (HIP events on the solver stream)
-    s.timing(True)
-    nt_k = min(args.steps, 32)
-    s.run(nt_k, ntime0=args.warmup + args.steps)
-    stage_ms, total_ms, nst = s.get_timing()
-    s.timing(False)
-    avg_ms = stage_ms / max(nst, 1)
-    achieved = ncell * BYTES_PER_CELL_LAUNCH / (avg_ms * 1e-3) / 1e9
-    log(f"RK stage (k_stage): {nst} stages, mean {avg_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic; "
-        f"stage kernels {stage_ms / total_ms * 100:.1f}% of step time")
+    log(f"RK stage (k_stage): mean {mean_us:.1f} us, {achieved:.0f} GB/s algorithmic ({bpc / 4:.0f} B per cell); "
+        f"stage kernels {share * 100:.1f}% of step time")
     traffic = None
-    if args.traffic and os.path.exists(args.traffic) and (args.ni, args.nj, args.nk) == (400, 400, 200):
-        with open(args.traffic) as f:
+    tpath = args.traffic or traffic_json(wl)
+    if os.path.exists(tpath) and (args.ni, args.nj, args.nk) == (400, 400, 200):
+        with open(tpath) as f:
             tj = json.load(f)
         # only a summary of the current stage kernel counts
         if "k_stage" in tj.get("kernels", {}):
             traffic = tj.get("bytes_per_launch")
-    s.close()
+    flat = None
+    if wl == "hills" and not args.no_flat:
+        mf = models.gradient_400x400x200(**kw)
+        fv, _, fus, fach, _ = time_solver(mf, args.steps, args.warmup, BYTES_PER_CELL_UPDATE["flat"])
+        log(f"flat variant (mode S): {fv / 1e9:.3f} G cell-updates/s, stage {fus:.1f} us, {fach:.0f} GB/s")
+        flat = {"workload": f"{mf.name}: the same model on the spherical grid without hills (mode S, the "
+                            "reference's scheme)",
+                "value": fv, "unit": "cell-updates/s", "mean_stage_us": fus,
+                "roofline_frac": fach / HBM_PEAK_GBS, "bytes_per_cell_update": BYTES_PER_CELL_UPDATE["flat"]}
+        del mf
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -324,16 +362,21 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{m.name}: plain FP32 solver, {m.ni}x{m.nj}x{m.nk} cells, h=0.5 km, "
-                               "flat free surface (spherical reference scheme), 12-pt CFS-PML on 5 sides, "
-                               "Ricker 1 Hz force", "grid": [m.ni, m.nj, m.nk], "dt": m.stept,
-                   "hbm_fraction_whole_step": value * BYTES_PER_CELL_UPDATE / (HBM_PEAK_GBS * 1e9)},
+        "config": {"workload": (f"{m.name}: plain FP32 solver, {m.ni}x{m.nj}x{m.nk} cells, h=0.5 km, "
+                                + ("terrain-following grid under 20 seeded Gaussian hills (0-2 km, sigma 5-20 km; "
+                                   "mode C, 3-D metric)" if wl == "hills" else
+                                   "flat free surface (spherical reference scheme, mode S)")
+                                + ", 12-pt CFS-PML on 5 sides, Ricker 1 Hz force"),
+                   "grid": [m.ni, m.nj, m.nk], "dt": m.stept, "mode": "C" if wl == "hills" else "S",
+                   "bytes_per_cell_update": bpc,
+                   "hbm_fraction_whole_step": value * bpc / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "RK stage = k_stage (operator sweep, CFS-PML and RK update; velocity and "
                                "stress halves as two wave groups of one block)",
-                     "mean_launch_us": avg_ms * 1e3},
+                     "mean_launch_us": mean_us},
         "cpu_baseline": cpu,
+        "flat_spherical": flat,
         "pipeline": pipe,
         "accuracy": acc,
     }

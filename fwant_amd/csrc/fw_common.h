@@ -168,12 +168,20 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 __device__ __forceinline__ float bf16_val(uint16_t c) { return __uint_as_float((uint32_t)c << 16); }
 __device__ __forceinline__ uint16_t fp16_rne(float x) { return __half_as_ushort(__float2half_rn(x)); }
 __device__ __forceinline__ float fp16_val(uint16_t c) { return __half2float(__ushort_as_half(c)); }
-// the factor applied before rounding, and the stored scale, for a block maximum amax
+// the factor applied before rounding, and the stored scale, for a block
+// maximum amax.  A block whose maximum is below SGT_TINY = 2^-100 is stored
+// as zeros with scale 0: 32768 / amax (or 1 / amax) would overflow for the
+// smallest maxima (the numerical precursors ahead of a wavefront reach
+// subnormal values), and x * inf gives NaN codes.
+constexpr float SGT_TINY = 7.888609052210118e-31f;  // 2^-100
 __device__ __forceinline__ float sgt_inv(int fmt, float amax) {
-  if (!(amax > 0.f)) return 0.f;
+  if (!(amax >= SGT_TINY)) return 0.f;
   return fmt == SGT_FP16 ? 32768.0f / amax : 1.0f / amax;
 }
-__device__ __forceinline__ float sgt_scale(int fmt, float amax) { return fmt == SGT_FP16 ? amax * (1.0f / 32768.0f) : amax; }
+__device__ __forceinline__ float sgt_scale(int fmt, float amax) {
+  if (!(amax >= SGT_TINY)) return 0.f;
+  return fmt == SGT_FP16 ? amax * (1.0f / 32768.0f) : amax;
+}
 __device__ __forceinline__ uint16_t sgt_code(int fmt, float x, float inv) {
   return fmt == SGT_FP16 ? fp16_rne(x * inv) : bf16_rne(x * inv);
 }

@@ -1326,13 +1326,20 @@ int fw_init(fw_handle h) {
     const long st[3] = {1, nx, (long)nx * ny};
     const int ext[3] = {nx, ny, nz};
     for (int ax = 0; ax < 3; ++ax) {
-      for (size_t u = 0; u < n; ++u) {
-        const int c = (int)((u / st[ax]) % ext[ax]);
-        uint8_t b = 0;
-        for (int d = -3; d <= 3; ++d)
-          if (c + d >= 0 && c + d < ext[ax]) b |= bad[u + d * st[ax]];
-        tmp[u] = b;
-      }
+      // each line along axis ax: a sliding count of bad cells over +-3
+      const int o1 = ax == 0 ? 1 : 0, o2 = ax == 2 ? 1 : 2;  // the two other axes
+      for (int b = 0; b < ext[o2]; ++b)
+        for (int a = 0; a < ext[o1]; ++a) {
+          const size_t base = (size_t)a * st[o1] + (size_t)b * st[o2];
+          const int L = ext[ax];
+          int cnt = 0;
+          for (int c = 0; c < 3 && c < L; ++c) cnt += bad[base + c * st[ax]];
+          for (int c = 0; c < L; ++c) {
+            if (c + 3 < L) cnt += bad[base + (c + 3) * st[ax]];
+            if (c - 4 >= 0) cnt -= bad[base + (c - 4) * st[ax]];
+            tmp[base + c * st[ax]] = cnt > 0;
+          }
+        }
       bad.swap(tmp);
     }
     for (int n6 = 0; n6 < 6; ++n6) {

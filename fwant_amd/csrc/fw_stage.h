@@ -490,13 +490,23 @@ __device__ __forceinline__ float ld(const float* __restrict__ p, const HaloSlot&
 // cell at offset a on plane kk, tau = sigma . N with N = (p, q, 1):
 //   tau_z = Tzz + p Txz + q Tyz, tau_x = Txz + p Txx + q Txy, tau_y = Tyz + p Txy + q Tyy;
 // above a free surface the imaged planes hold tau itself in Tzz Txz Tyz.
-__device__ __forceinline__ void tau_at(const StageArgs& g, const float* __restrict__ pTxx, const float* __restrict__ pTyy,
+struct Sig6 {
+  float s[6];  // Txx Tyy Txy Tzz Txz Tyz of one cell
+  float p, q;
+};
+__device__ __forceinline__ Sig6 sig_at(const StageArgs& g, const float* __restrict__ pTxx, const float* __restrict__ pTyy,
                                        const float* __restrict__ pTxy, const float* __restrict__ pTzz,
-                                       const float* __restrict__ pTxz, const float* __restrict__ pTyz, unsigned a, bool ok,
-                                       int kk, float& tz, float& tx, float& ty) {
-  const float xx = ldv(pTxx, a, ok), yy = ldv(pTyy, a, ok), xy = ldv(pTxy, a, ok);
-  const float zz = ldv(pTzz, a, ok), xz = ldv(pTxz, a, ok), yz = ldv(pTyz, a, ok);
-  const float p = ldv(g.cp, a, ok), q = ldv(g.cq, a, ok);
+                                       const float* __restrict__ pTxz, const float* __restrict__ pTyz, unsigned a,
+                                       bool ok) {
+  Sig6 r;
+  r.s[0] = ldv(pTxx, a, ok); r.s[1] = ldv(pTyy, a, ok); r.s[2] = ldv(pTxy, a, ok);
+  r.s[3] = ldv(pTzz, a, ok); r.s[4] = ldv(pTxz, a, ok); r.s[5] = ldv(pTyz, a, ok);
+  r.p = ldv(g.cp, a, ok);
+  r.q = ldv(g.cq, a, ok);
+  return r;
+}
+__device__ __forceinline__ void tau_of(const StageArgs& g, const Sig6& v, int kk, float& tz, float& tx, float& ty) {
+  const float xx = v.s[0], yy = v.s[1], xy = v.s[2], zz = v.s[3], xz = v.s[4], yz = v.s[5], p = v.p, q = v.q;
   if (g.freenode && kk > g.nk2) {
     tz = zz; tx = xz; ty = yz;
   } else {
@@ -505,6 +515,16 @@ __device__ __forceinline__ void tau_at(const StageArgs& g, const float* __restri
     ty = yz + p * xy + q * yy;
   }
 }
+// Mode C FIFO of stage-input stresses: a thread keeps the six stresses of
+// its cells from the window's lead plane until that plane becomes the
+// centre plane (3 planes later for F, 1 for B) in LDS words of its own,
+// instead of loading them again (by then the lines have often left L2).
+// Layout [plane mod FD][field][row][thread]: consecutive threads hit
+// consecutive banks.  Only for one row per thread (the LDS budget).
+template <int ZF>
+struct Fifo {
+  static constexpr int FD = ZF ? 4 : 2;
+};
 
 // LDS planes of the two groups (per buffer): velocity group Txx Txy Txz Tyy
 // Tyz (with halos) and Tzz (centres only); stress group Vx Vy Vz.
@@ -526,7 +546,8 @@ enum { LTXX = 0, LTXY, LTXZ, LTYY, LTYZ, LTZZ };
 // 1/r is per cell; the plane centres of Tzz Txz Tyz are loaded separately.
 // ---------------------------------------------------------------------------
 template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML, bool CURV>
-__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
+__device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy,
+                                         float* fifo) {
   // RS: rows per row group (the stride between a thread's rows)
   using G = TileGeom<BYT, RY, TW>;
   constexpr int RS = BYT * (TX / TW), TY = G::TY, LW = G::LW, PL = G::PL;
@@ -563,6 +584,12 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
   float w[RY][3][5];  // Tzz Txz Tyz (CURV: tau_z tau_x tau_y)
   float cTxx[RY], cTyy[RY], cTxy[RY];
   float cTzz[RY], cTxz[RY], cTyz[RY];  // CURV: plane centres of the window fields
+  // CURV with one row per thread: stresses from the lead plane to the centre
+  // plane through the thread's own LDS words (FIFO)
+  constexpr bool FIFO = CURV && RY == 1;
+  constexpr int FD = Fifo<ZF>::FD, GT = TX * BYT;
+  const int ftid = threadIdx.x + TX * wy;
+  auto fword = [&](int plane, int f) -> float* { return fifo + ((plane & (FD - 1)) * 6 + f) * GT + ftid; };
   bool ok[RY], comp[RY], fluid[RY];
   unsigned cc[RY];
   float et[RY];
@@ -579,18 +606,29 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
     for (int s = 0; s < 5; ++s) {
       const int o = ZF ? s - 1 : s - 3;
       if constexpr (CURV) {
-        tau_at(g, pTxx, pTyy, pTxy, pTzz, pTxz, pTyz, c0 + o * (int)sz, ok[q], k0 + o, w[q][0][s], w[q][1][s],
-               w[q][2][s]);
+        const Sig6 v = sig_at(g, pTxx, pTyy, pTxy, pTzz, pTxz, pTyz, c0 + o * (int)sz, ok[q]);
+        tau_of(g, v, k0 + o, w[q][0][s], w[q][1][s], w[q][2][s]);
+        if constexpr (FIFO) {
+          if (o == 0) {
+            cTxx[q] = v.s[0]; cTyy[q] = v.s[1]; cTxy[q] = v.s[2];
+            cTzz[q] = v.s[3]; cTxz[q] = v.s[4]; cTyz[q] = v.s[5];
+          } else if (o > 0) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) *fword(k0 + o, f) = v.s[f];
+          }
+        }
       } else {
         w[q][0][s] = ldv(pTzz, c0 + o * (int)sz, ok[q]);
         w[q][1][s] = ldv(pTxz, c0 + o * (int)sz, ok[q]);
         w[q][2][s] = ldv(pTyz, c0 + o * (int)sz, ok[q]);
       }
     }
-    cTxx[q] = ldv(pTxx, c0, ok[q]);
-    cTyy[q] = ldv(pTyy, c0, ok[q]);
-    cTxy[q] = ldv(pTxy, c0, ok[q]);
-    if constexpr (CURV) {
+    if constexpr (!FIFO) {
+      cTxx[q] = ldv(pTxx, c0, ok[q]);
+      cTyy[q] = ldv(pTyy, c0, ok[q]);
+      cTxy[q] = ldv(pTxy, c0, ok[q]);
+    }
+    if constexpr (CURV && !FIFO) {
       cTzz[q] = ldv(pTzz, c0, ok[q]);
       cTxz[q] = ldv(pTxz, c0, ok[q]);
       cTyz[q] = ldv(pTyz, c0, ok[q]);
@@ -675,24 +713,29 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
     // prefetch plane k+1 (the last plane re-reads plane k: every address valid)
     float nTxx[RY], nTyy[RY], nTxy[RY], lead[RY][3];
     float nTzz[RY], nTxz[RY], nTyz[RY];
+    Sig6 lsig[RY];  // CURV: the lead plane's stresses and slopes
+    const int kl = (more ? k + 1 : k) + (ZF ? 3 : 1);
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
       const unsigned c = cc[q] + kn;
       const unsigned l = c + (ZF ? 3u : 1u) * sz;
       if constexpr (CURV) {
-        tau_at(g, pTxx, pTyy, pTxy, pTzz, pTxz, pTyz, l, ok[q], (more ? k + 1 : k) + (ZF ? 3 : 1), lead[q][0],
-               lead[q][1], lead[q][2]);
-        nTzz[q] = ldv(pTzz, c, ok[q]);
-        nTxz[q] = ldv(pTxz, c, ok[q]);
-        nTyz[q] = ldv(pTyz, c, ok[q]);
+        lsig[q] = sig_at(g, pTxx, pTyy, pTxy, pTzz, pTxz, pTyz, l, ok[q]);
+        if constexpr (!FIFO) {
+          nTzz[q] = ldv(pTzz, c, ok[q]);
+          nTxz[q] = ldv(pTxz, c, ok[q]);
+          nTyz[q] = ldv(pTyz, c, ok[q]);
+        }
       } else {
         lead[q][0] = ldv(pTzz, l, ok[q]);
         lead[q][1] = ldv(pTxz, l, ok[q]);
         lead[q][2] = ldv(pTyz, l, ok[q]);
       }
-      nTxx[q] = ldv(pTxx, c, ok[q]);
-      nTyy[q] = ldv(pTyy, c, ok[q]);
-      nTxy[q] = ldv(pTxy, c, ok[q]);
+      if constexpr (!FIFO) {
+        nTxx[q] = ldv(pTxx, c, ok[q]);
+        nTyy[q] = ldv(pTyy, c, ok[q]);
+        nTxy[q] = ldv(pTxy, c, ok[q]);
+      }
     }
     const float nhxTxx = ld(pTxx, hx, kn), nhxTxy = ld(pTxy, hx, kn), nhxTxz = ld(pTxz, hx, kn);
     const float nhyTxy = ld(pTxy, hy, kn), nhyTyy = ld(pTyy, hy, kn), nhyTyz = ld(pTyz, hy, kn);
@@ -772,19 +815,29 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         }
       }
       // advance the window and the plane registers
+      if constexpr (CURV) tau_of(g, lsig[q], kl, lead[q][0], lead[q][1], lead[q][2]);
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
 #pragma unroll
         for (int sl = 0; sl < 4; ++sl) w[q][f][sl] = w[q][f][sl + 1];
         w[q][f][4] = lead[q][f];
       }
-      cTxx[q] = nTxx[q];
-      cTyy[q] = nTyy[q];
-      cTxy[q] = nTxy[q];
-      if constexpr (CURV) {
-        cTzz[q] = nTzz[q];
-        cTxz[q] = nTxz[q];
-        cTyz[q] = nTyz[q];
+      if constexpr (FIFO) {
+        // the lead plane's stresses in, the next centre plane's out (distinct
+        // words: the planes differ by less than FD)
+#pragma unroll
+        for (int f = 0; f < 6; ++f) *fword(kl, f) = lsig[q].s[f];
+        cTxx[q] = *fword(k + 1, 0); cTyy[q] = *fword(k + 1, 1); cTxy[q] = *fword(k + 1, 2);
+        cTzz[q] = *fword(k + 1, 3); cTxz[q] = *fword(k + 1, 4); cTyz[q] = *fword(k + 1, 5);
+      } else {
+        cTxx[q] = nTxx[q];
+        cTyy[q] = nTyy[q];
+        cTxy[q] = nTxy[q];
+        if constexpr (CURV) {
+          cTzz[q] = nTzz[q];
+          cTxz[q] = nTxz[q];
+          cTyz[q] = nTyz[q];
+        }
       }
     }
     hxTxx = nhxTxx; hxTxy = nhxTxy; hxTxz = nhxTxz;
@@ -1080,11 +1133,14 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
 // LDS of one block: the velocity group's planes, then the stress group's,
 // each sized for the larger of the wide and narrow tiles.  Each group's region ends with TX spare words (halo writes of lanes
 // without a halo slot).
-template <int BYT, int RY>
+template <int BYT, int RY, bool CURV>
 struct StageLds {
   static constexpr int VW = 2 * VNL * TileGeom<BYT, RY, TX>::PL, VN = 2 * VNL * TileGeom<BYT, RY, TXN>::PL;
   static constexpr int SW = 2 * SNL * TileGeom<BYT, RY, TX>::PL, SN = 2 * SNL * TileGeom<BYT, RY, TXN>::PL;
-  static constexpr int VOFF = (VW > VN ? VW : VN) + TX, SIZE = VOFF + (SW > SN ? SW : SN) + TX;
+  static constexpr int VOFF = (VW > VN ? VW : VN) + TX, FOFF = VOFF + (SW > SN ? SW : SN) + TX;
+  // mode C stress FIFO of the velocity group (one row per thread)
+  static constexpr int FF = (CURV && RY == 1) ? Fifo<1>::FD * 6 * TX * BYT : 0;
+  static constexpr int SIZE = FOFF + FF;
 };
 
 // One launch per RK stage: blocks of 2 x BYT waves, the velocity group
@@ -1094,7 +1150,7 @@ struct StageLds {
 // blocks per CU).
 template <int XF, int YF, int ZF, int FORM, int BYT, int RY, bool CURV>
 __global__ __launch_bounds__(2 * TX * BYT) __attribute__((amdgpu_waves_per_eu(4))) void k_stage(StageArgs g) {
-  using L = StageLds<BYT, RY>;
+  using L = StageLds<BYT, RY, CURV>;
   __shared__ float sh[L::SIZE];
   const ItemPos ip = item_pos(g);
   const int it_ = ip.item, slot = g.slot0 + ip.member;
@@ -1105,11 +1161,11 @@ __global__ __launch_bounds__(2 * TX * BYT) __attribute__((amdgpu_waves_per_eu(4)
   float* const ss = sh + L::VOFF;
   if (wave < BYT) {
     if (it_ < g.p1)
-      vel_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, sv, ss, wave);
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, sv, ss, wave, sh + L::FOFF);
     else if (it_ < g.p2)
-      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, true, CURV>(g, it, slot, sv, ss, wave);
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, true, CURV>(g, it, slot, sv, ss, wave, sh + L::FOFF);
     else
-      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, sv, ss, wave);
+      vel_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, sv, ss, wave, sh + L::FOFF);
   } else {
     if (it_ < g.p1)
       str_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, ss, sv, wave - BYT);

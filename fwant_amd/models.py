@@ -17,6 +17,7 @@ Presets (sizes and values of the named workloads):
   small_test(...)          a reduced heterogeneous case for parity tests
   hills_400x400x200()      gradient_400x400x200 on a terrain-following grid
                            under 20 seeded Gaussian hills (mode C)
+  hills_256x256x128()      the batched-sources model: 256x256x128 under hills
   hills_test(...)          small_test under steep hills (mode C parity)
 """
 from dataclasses import dataclass, field
@@ -371,6 +372,16 @@ def hills_400x400x200(nt=3000, ni=400, nj=400, nk=200, h=500.0):
     topo = gaussian_hills(m.x, m.y, R_EARTH, 20, seed=2)
     n = with_topography(m, topo)
     n.name = f"hills_{ni}x{nj}x{nk}"
+    return n
+
+
+def hills_256x256x128(nt=4000):
+    """The batched-sources configuration's curvilinear model: the gradient
+    model at 256x256x128 (h = 0.5 km, seed 4) under 20 Gaussian hills
+    (default_rng(2), 0-2 km, sigma 5-20 km), tapered over the CFS-PML shell."""
+    m = gradient_400x400x200(nt=nt, ni=256, nj=256, nk=128, h=500.0, seed=4)
+    n = with_topography(m, gaussian_hills(m.x, m.y, R_EARTH, 20, seed=2))
+    n.name = "hills_256x256x128"
     return n
 
 

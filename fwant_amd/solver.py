@@ -19,7 +19,7 @@ from ._lib import FwConfig, check, fptr, iptr, dptr
 
 
 class Solver:
-    def __init__(self, model, metric=None, max_records=None, snapshots=(), batch=None):
+    def __init__(self, model, metric=None, max_records=None, snapshots=(), batch=None, metric3d_out=False):
         """model: fwant_amd.models.Model.  metric: None -> build on the GPU in
         FP64 (fw_build_metric); or a dict with xsin, xcot, xi_x, eta_y, zeta_z
         (float32) as read from metric_mpi*.nc.  snapshots: iterable of dicts
@@ -27,7 +27,9 @@ class Solver:
         optionally format ("fp32", "bf16" or "fp16": 16-bit codes with a
         per-16^3-block FP32 scale).  batch: None, or a Batch (member of each
         force / moment source of the model and the steps each member runs):
-        the members advance together in the same launches."""
+        the members advance together in the same launches.  A model with r3
+        (a terrain-following grid) runs mode C (fw_build_metric3d);
+        metric3d_out keeps its FP32 metric (p, q, zz, rr) in self.metric3d."""
         L = _lib.lib()
         self.L = L
         self.model = model
@@ -66,10 +68,14 @@ class Solver:
             # mode C: the terrain-following grid's 3-D metric, built on the GPU in FP64
             r3 = np.asfortranarray(model.r3, np.float64).ravel(order="F")
             shp = (self.nx, self.ny, self.nz)
-            m3 = {k: np.zeros(self.nx * self.ny * self.nz, np.float32) for k in ("p", "q", "zz", "rr")}
-            check(L.fw_build_metric3d(h, dptr(x), dptr(y), dptr(r3), fptr(m3["p"]), fptr(m3["q"]), fptr(m3["zz"]),
-                                      fptr(m3["rr"])), h, "fw_build_metric3d")
-            self.metric3d = {k: v.reshape(shp, order="F") for k, v in m3.items()}
+            if metric3d_out:
+                m3 = {k: np.zeros(self.nx * self.ny * self.nz, np.float32) for k in ("p", "q", "zz", "rr")}
+                check(L.fw_build_metric3d(h, dptr(x), dptr(y), dptr(r3), fptr(m3["p"]), fptr(m3["q"]),
+                                          fptr(m3["zz"]), fptr(m3["rr"])), h, "fw_build_metric3d")
+                self.metric3d = {k: v.reshape(shp, order="F") for k, v in m3.items()}
+            else:
+                nul = C.POINTER(C.c_float)()
+                check(L.fw_build_metric3d(h, dptr(x), dptr(y), dptr(r3), nul, nul, nul, nul), h, "fw_build_metric3d")
         lam = np.asfortranarray(model.lam, np.float32).ravel(order="F")
         mu = np.asfortranarray(model.mu, np.float32).ravel(order="F")
         rho = np.asfortranarray(model.rho, np.float32).ravel(order="F")

@@ -1377,7 +1377,8 @@ void ORC(scatter_rotate)(long npt, int knt, real *X, real *Y, real *Z, const rea
  * stores FP32).  Per 16^3 block of snapshot points and per slab: amax =
  * largest |x|; BF16 codes of x/amax (scale amax) or FP16 codes of
  * x*32768/amax (scale amax/32768), round to nearest even; x' = code*scale.
- * Always float arithmetic, whatever REAL is.  Pinned by
+ * A block with amax below 2^-100 is stored as zeros with scale 0 (the
+ * factor would overflow).  Always float arithmetic, whatever REAL is.  Pinned by
  * tests/test_oracle_sgt_codec.py: conversions against numpy float16 and
  * torch bfloat16, block scales against numpy maxima, restore error bounds.
  * ====================================================================== */
@@ -1436,8 +1437,10 @@ void ORC(sgt_encode)(int fmt, int c0, int c1, int c2, long nslab, const float *x
             for (int b = b1 * 16; b < c1 && b < b1 * 16 + 16; b++)
               for (int a = b0 * 16; a < c0 && a < b0 * 16 + 16; a++)
                 amax = fmaxf(amax, fabsf(x[sl * npts + a + (long)c0 * (b + (long)c1 * d)]));
-          const float inv = amax > 0.0f ? (fmt == 2 ? 32768.0f / amax : 1.0f / amax) : 0.0f;
-          scales[sl * nblk + blk] = fmt == 2 ? amax * (1.0f / 32768.0f) : amax;
+          /* maxima below 2^-100 (where 32768 / amax would overflow) store zeros */
+          const int tiny = !(amax >= 7.888609052210118e-31f);
+          const float inv = tiny ? 0.0f : (fmt == 2 ? 32768.0f / amax : 1.0f / amax);
+          scales[sl * nblk + blk] = tiny ? 0.0f : (fmt == 2 ? amax * (1.0f / 32768.0f) : amax);
           for (int d = b2 * 16; d < c2 && d < b2 * 16 + 16; d++)
             for (int b = b1 * 16; b < c1 && b < b1 * 16 + 16; b++)
               for (int a = b0 * 16; a < c0 && a < b0 * 16 + 16; a++) {

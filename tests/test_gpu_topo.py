@@ -43,7 +43,7 @@ def test_metric3d_matches_oracle(gpu_lib):
     from fwant_amd.solver import Solver
     from oracle.pyoracle import build_metric3d
     m = models.hills_test(nt=1)
-    s = Solver(m)
+    s = Solver(m, metric3d_out=True)
     o = build_metric3d(m)
     for k in ("p", "q", "zz", "rr"):
         g = s.metric3d[k]
@@ -64,7 +64,7 @@ def test_flat_terrain_matches_the_spherical_scheme(gpu_lib):
     from fwant_amd.solver import Solver
     m = models.small_test(ni=40, nj=36, nk=28, nt=48)
     mf = models.with_topography(m, np.zeros((m.nx, m.ny)))
-    a, b = Solver(m), Solver(mf)
+    a, b = Solver(m), Solver(mf, metric3d_out=True)
     assert np.all(b.metric3d["p"] == 0.0) and np.all(b.metric3d["q"] == 0.0)
     assert np.array_equal(b.metric3d["zz"], np.broadcast_to(a.metric["zeta_z"], b.metric3d["zz"].shape))
     a.run(48)
@@ -89,7 +89,7 @@ def test_hills_parity(gpu_lib, monkeypatch, tile):
     n = m.nt
     print(f"tile={tile} grid={m.ni}x{m.nj}x{m.nk} steps={n} PML {m.abs_number} "
           f"hills max {np.max(m.r3[:, :, m.nk2] - m.z[m.nk2]):.0f} m", flush=True)
-    s = Solver(m)
+    s = Solver(m, metric3d_out=True)
     s.run(n)
     g = s.receivers()
     o = OracleSolver(m, 32, metric=s.metric, metric3d=s.metric3d)
@@ -107,7 +107,7 @@ def test_hills_random_state_parity(gpu_lib):
     from oracle.pyoracle import OracleSolver
     m = models.hills_test(ni=76, nj=40, nk=28, nt=16, seed=9)
     print(f"grid={m.ni}x{m.nj}x{m.nk} PML {m.abs_number}, 16 steps from a random state", flush=True)
-    s = Solver(m, max_records=32)
+    s = Solver(m, max_records=32, metric3d_out=True)
     o = OracleSolver(m, 32, metric=s.metric, metric3d=s.metric3d)
     rng = np.random.default_rng(12)
     for f in range(9):

@@ -86,3 +86,33 @@ def test_block_scales_and_restore_error(fmt):
     rel = np.linalg.norm(y - x) / np.linalg.norm(x)
     print(f"{fmt}: restore rel-L2 {rel:.2e}")
     assert rel < (2e-3 if fmt == "bf16" else 3e-4)
+
+
+@pytest.mark.parametrize("fmt", ["bf16", "fp16"])
+def test_tiny_blocks_store_zeros(fmt):
+    """Blocks whose largest magnitude is below 2^-100 (the numerical
+    precursors ahead of a wavefront reach subnormal values) are stored as
+    zeros with scale 0: 32768 / amax (FP16) or 1 / amax (BF16) would
+    overflow and turn zeros into NaN codes.  Blocks just above keep their
+    restore bound."""
+    subc = (32, 16, 16)  # two blocks
+    npts = int(np.prod(subc))
+    x = np.zeros((4, npts), np.float32)
+    v = x.reshape(4, 16, 16, 32)
+    rng = np.random.default_rng(9)
+    for s, amax in enumerate((1e-38, 1e-34, 2e-31, 1e-29)):
+        blk = rng.standard_normal((16, 16, 16)).astype(np.float32)
+        blk[0, 0, 0] = 0.0
+        v[s, :, :, :16] = blk / np.abs(blk).max() * np.float32(amax)
+    codes, scales = sgt_encode(fmt, x, subc)
+    y = sgt_decode(fmt, codes, scales, subc)
+    assert np.isfinite(y).all()
+    tiny = np.float32(2.0 ** -100)
+    for s in range(4):
+        a = np.abs(v[s, :, :, :16]).max()
+        if a < tiny:
+            assert scales[s, 0] == 0.0 and np.all(y[s] == 0.0)
+        else:
+            err = np.abs(y[s].astype(np.float64) - x[s])
+            assert err.max() <= 2.0 ** -7 * a
+    assert np.all(scales[:, 1] == 0.0)  # the all-zero block

@@ -275,6 +275,6 @@ def _pml_flat_violations(m):
 
 def test_topography_is_flat_over_the_pml_shell():
     for m in (models.hills_test(nt=1), models.hills_test(ni=84, nj=40, nk=28, nt=1),
-              models.hills_400x400x200(nt=1)):
+              models.hills_400x400x200(nt=1), models.hills_256x256x128(nt=1)):
         assert _pml_flat_violations(m) == 0, m.name
         assert np.max(m.r3[:, :, m.nk2] - m.z[m.nk2]) > 500.0
