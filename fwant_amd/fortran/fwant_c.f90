@@ -9,7 +9,7 @@ module fwant_c
   implicit none
   private
   public :: fw_config, fw_check, fw_error_message
-  public :: fw_create, fw_destroy, fw_last_error, fw_dims, fw_set_grid, fw_build_metric, fw_set_media
+  public :: fw_create, fw_destroy, fw_last_error, fw_dims, fw_set_grid, fw_build_metric, fw_build_metric3d, fw_set_media
   public :: fw_set_force, fw_set_moment, fw_set_receivers, fw_add_snapshot, fw_init, fw_get_pml_profiles
   public :: fw_step, fw_stage, fw_run, fw_reference_steps, fw_sync, fw_get_receivers, fw_get_snapshot
   public :: fw_get_wavefield, fw_set_wavefield, fw_enable_timing, fw_get_timing
@@ -59,6 +59,15 @@ module fwant_c
       type(c_ptr), value :: h
       real(c_double), intent(in) :: x(*), y(*), z(*)
       real(c_float), intent(out) :: xsin(*), xcot(*), xi_x(*), eta_y(*), zeta_z(*)
+    end function
+    ! mode C (terrain-following grid): the radius r(nx*ny*nz) of every node;
+    ! the outputs p, q, zeta_r, 1/r are optional (c_null_ptr or c_loc of an
+    ! nx*ny*nz real(c_float) array)
+    integer(c_int) function fw_build_metric3d(h, x, y, r, p, q, zeta_r, rr) bind(C, name="fw_build_metric3d")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: x(*), y(*), r(*)
+      type(c_ptr), value :: p, q, zeta_r, rr
     end function
     integer(c_int) function fw_set_media(h, lam, mu, rho) bind(C, name="fw_set_media")
       import :: c_int, c_ptr, c_float
