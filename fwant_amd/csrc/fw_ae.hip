@@ -204,7 +204,9 @@ __global__ void k_blocks(const float* __restrict__ rec, int nc, int c0, int c1, 
     if ((int)threadIdx.x < o) sh[threadIdx.x] = fmaxf(sh[threadIdx.x], sh[threadIdx.x + o]);
     __syncthreads();
   }
-  const float s = sh[0];
+  // a component whose maximum is below SGT_TINY is a zero block (1 / s would
+  // overflow for subnormal maxima and turn its zeros into NaN)
+  const float s = sh[0] >= fw::SGT_TINY ? sh[0] : 0.f;
   const float inv = s > 0.f ? 1.0f / s : 0.f;
   for (int t = threadIdx.x; t < 4096; t += blockDim.x) dst[t] *= inv;
   if (threadIdx.x == 0) scale[((long)r * nblk + blk) * nc + c] = s;
