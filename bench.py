@@ -19,9 +19,11 @@ Prints ONE JSON line:
              with HIP events on the solver's stream, against the 8 TB/s HBM
              peak.  Algorithmic bytes per cell and stage: mode S 156 B (the
              average of 120/192/192/120 B over the 4 RK stages of the
-             3-register RK4 form: W, media, m, t); mode C 180 B (the same
-             plus the 6 metric values of a cell that the stage reads:
-             slopes p, q, zeta_r, 1/r and the zeta derivatives of p, q).
+             3-register RK4 form: W, media, m, t); mode C 196 B, the
+             per-unit figure of the measurement contract (156 B plus a
+             10-value curvilinear metric).  The terrain-following scheme
+             reads 4 metric values per cell and stage (slopes p, q,
+             zeta_r, 1/r): 172 B, reported as frac_scheme_bytes.
              traffic = HBM bytes per stage from rocprofv3 FETCH_SIZE /
              WRITE_SIZE runs, scaled by the same counters on a plain copy
              (summarised by --summarize-pmc into
@@ -49,9 +51,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic FP32 bytes per cell-update (4 RK stages): mode S 156 values,
-# mode C 156 + 4 x 6 metric values
-BYTES_PER_CELL_UPDATE = {"flat": 624.0, "hills": 720.0}
+# algorithmic FP32 bytes per cell-update (4 RK stages): mode S 156 values;
+# mode C 156 + 4 x 10 metric values (the contract's general curvilinear
+# metric); the terrain-following scheme's own minimum 156 + 4 x 4
+BYTES_PER_CELL_UPDATE = {"flat": 624.0, "hills": 784.0}
+SCHEME_BYTES_PER_CELL_UPDATE = {"flat": 624.0, "hills": 688.0}
 WORKLOAD_NAME = {"flat": "gradient_400x400x200", "hills": "hills_400x400x200"}
 
 
@@ -368,10 +372,12 @@ def main():
                                    "flat free surface (spherical reference scheme, mode S)")
                                 + ", 12-pt CFS-PML on 5 sides, Ricker 1 Hz force"),
                    "grid": [m.ni, m.nj, m.nk], "dt": m.stept, "mode": "C" if wl == "hills" else "S",
-                   "bytes_per_cell_update": bpc,
+                   "bytes_per_cell_update": bpc, "scheme_bytes_per_cell_update": SCHEME_BYTES_PER_CELL_UPDATE[wl],
                    "hbm_fraction_whole_step": value * bpc / (HBM_PEAK_GBS * 1e9)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_cell_stage": bpc / 4,
+                     "frac_scheme_bytes": achieved * SCHEME_BYTES_PER_CELL_UPDATE[wl] / bpc / HBM_PEAK_GBS,
                      "kernel": "RK stage = k_stage (operator sweep, CFS-PML and RK update; velocity and "
                                "stress halves as two wave groups of one block)",
                      "mean_launch_us": mean_us},

@@ -515,15 +515,22 @@ __device__ __forceinline__ void tau_of(const StageArgs& g, const Sig6& v, int kk
     ty = yz + p * xy + q * yy;
   }
 }
-// Mode C FIFO of stage-input stresses: a thread keeps the six stresses of
-// its cells from the window's lead plane until that plane becomes the
-// centre plane (3 planes later for F, 1 for B) in LDS words of its own,
-// instead of loading them again (by then the lines have often left L2).
-// Layout [plane mod FD][field][row][thread]: consecutive threads hit
-// consecutive banks.  Only for one row per thread (the LDS budget).
-template <int ZF>
+// Mode C FIFOs (one row per thread: the LDS budget).
+// Stresses: a velocity-group thread keeps the six stresses of its cell from
+// the window's lead plane until that plane becomes the centre plane (3
+// planes later for F, 1 for B) in LDS words of its own, instead of loading
+// them again (by then the lines have often left L2).  Layout [plane mod
+// FD][field][thread]: consecutive threads hit consecutive banks.  FD = 3:
+// at the end of iteration k the thread reads plane k+1 before it writes
+// plane k+4 (F) into the same slot.
+// Slopes: the velocity-group thread also writes p and q of the lead plane:
+// the five planes of its z window stay there (FS = 5), from which it forms
+// F(p) and F(q) itself, and the stress-group thread of the same cell (same
+// thread index in its group) reads plane k after iteration k's barrier.  A
+// slot is rewritten (plane k+5 at the end of iteration k+1 for F) only
+// after the barrier of iteration k+1 that follows that read.
 struct Fifo {
-  static constexpr int FD = ZF ? 4 : 2;
+  static constexpr int FD = 3, FS = 5;
 };
 
 // LDS planes of the two groups (per buffer): velocity group Txx Txy Txz Tyy
@@ -587,9 +594,11 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
   // CURV with one row per thread: stresses from the lead plane to the centre
   // plane through the thread's own LDS words (FIFO)
   constexpr bool FIFO = CURV && RY == 1;
-  constexpr int FD = Fifo<ZF>::FD, GT = TX * BYT;
+  constexpr int FD = Fifo::FD, FS = Fifo::FS, GT = TX * BYT;
   const int ftid = threadIdx.x + TX * wy;
-  auto fword = [&](int plane, int f) -> float* { return fifo + ((plane & (FD - 1)) * 6 + f) * GT + ftid; };
+  auto fword = [&](int plane, int f) -> float* { return fifo + ((plane % FD) * 6 + f) * GT + ftid; };
+  // the slopes' FIFO for the stress group follows the stress FIFO
+  auto sword = [&](int plane, int f) -> float* { return fifo + (FD * 6 + (plane % FS) * 2 + f) * GT + ftid; };
   bool ok[RY], comp[RY], fluid[RY];
   unsigned cc[RY];
   float et[RY];
@@ -616,6 +625,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 #pragma unroll
             for (int f = 0; f < 6; ++f) *fword(k0 + o, f) = v.s[f];
           }
+          *sword(k0 + o, 0) = v.p;
+          *sword(k0 + o, 1) = v.q;
         }
       } else {
         w[q][0][s] = ldv(pTzz, c0 + o * (int)sz, ok[q]);
@@ -753,8 +764,10 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         const unsigned c = cc[q] + kz;
         mze[q] = ldv(g.czz, c, ok[q]);
         mrz[q] = ldv(g.crr, c, ok[q]);
-        mdp[q] = ldv(g.cdp, c, ok[q]);
-        mdq[q] = ldv(g.cdq, c, ok[q]);
+        if constexpr (!FIFO) {
+          mdp[q] = ldv(g.cdp, c, ok[q]);
+          mdq[q] = ldv(g.cdq, c, ok[q]);
+        }
       }
     }
     __syncthreads();
@@ -775,6 +788,17 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         const float Tzz = CURV ? cTzz[q] : zc<ZF>(w[q][0]), Txz = CURV ? cTxz[q] : zc<ZF>(w[q][1]),
                     Tyz = CURV ? cTyz[q] : zc<ZF>(w[q][2]);
         float DzTzz0, DzTxz0, DzTyz0;
+        if constexpr (FIFO) {
+          // F(p), F(q) from the slopes of the window's planes
+          float pw[5], qw[5];
+#pragma unroll
+          for (int sl = 0; sl < 5; ++sl) {
+            pw[sl] = *sword(k + (ZF ? sl - 1 : sl - 3), 0);
+            qw[sl] = *sword(k + (ZF ? sl - 1 : sl - 3), 1);
+          }
+          mdp[q] = dz5<ZF>(pw);
+          mdq[q] = dz5<ZF>(qw);
+        }
         if constexpr (CURV) {
           DzTzz0 = (dz5<ZF>(w[q][0]) - Txz * mdp[q] - Tyz * mdq[q]) * ze;
           DzTxz0 = (dz5<ZF>(w[q][1]) - Txx * mdp[q] - Txy * mdq[q]) * ze;
@@ -823,12 +847,15 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
         w[q][f][4] = lead[q][f];
       }
       if constexpr (FIFO) {
-        // the lead plane's stresses in, the next centre plane's out (distinct
-        // words: the planes differ by less than FD)
-#pragma unroll
-        for (int f = 0; f < 6; ++f) *fword(kl, f) = lsig[q].s[f];
+        // the next centre plane's stresses out, then the lead plane's in
+        // (for F the same slot), and the lead plane's slopes for the stress
+        // group
         cTxx[q] = *fword(k + 1, 0); cTyy[q] = *fword(k + 1, 1); cTxy[q] = *fword(k + 1, 2);
         cTzz[q] = *fword(k + 1, 3); cTxz[q] = *fword(k + 1, 4); cTyz[q] = *fword(k + 1, 5);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) *fword(kl, f) = lsig[q].s[f];
+        *sword(kl, 0) = lsig[q].p;
+        *sword(kl, 1) = lsig[q].q;
       } else {
         cTxx[q] = nTxx[q];
         cTyy[q] = nTyy[q];
@@ -860,7 +887,8 @@ __device__ __forceinline__ void vel_body(const StageArgs& g, const int4 it, int 
 // traction-rate condition (surf_solve; the source part is added by k_src).
 // ---------------------------------------------------------------------------
 template <int XF, int YF, int ZF, int FORM, int BYT, int RY, int TW, bool PML, bool CURV>
-__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy) {
+__device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int slot, float* own, const float* other, int wy,
+                                         const float* fifo) {
   using G = TileGeom<BYT, RY, TW>;
   constexpr int RS = BYT * (TX / TW), TY = G::TY, LW = G::LW, PL = G::PL;
   constexpr bool FULLS = AUX_FULL_STR && TW == TX;
@@ -998,17 +1026,28 @@ __device__ __forceinline__ void str_body(const StageArgs& g, const int4 it, int 
     // plane metric with the plane's other loads (used right after the barrier)
     const float ze1 = g.zeta_z[k], rz1 = g.rz[k];
     float mze[RY], mrz[RY], mp[RY], mq[RY];  // CURV: zeta_r, 1/r, slopes of the cell
+    // CURV with one row per thread: the slopes come from the velocity group's
+    // FIFO (written at its window lead), read after the barrier
+    constexpr bool SFIFO = CURV && RY == 1;
     if constexpr (CURV) {
 #pragma unroll
       for (int q = 0; q < RY; ++q) {
         const unsigned c = cc[q] + kz;
         mze[q] = ldv(g.czz, c, ok[q]);
         mrz[q] = ldv(g.crr, c, ok[q]);
-        mp[q] = ldv(g.cp, c, ok[q]);
-        mq[q] = ldv(g.cq, c, ok[q]);
+        if constexpr (!SFIFO) {
+          mp[q] = ldv(g.cp, c, ok[q]);
+          mq[q] = ldv(g.cq, c, ok[q]);
+        }
       }
     }
     __syncthreads();
+    if constexpr (SFIFO) {
+      constexpr int FD = Fifo::FD, FS = Fifo::FS, GT = TX * BYT;
+      const int ftid = threadIdx.x + TX * wy;
+      mp[0] = fifo[(FD * 6 + (k % FS) * 2 + 0) * GT + ftid];
+      mq[0] = fifo[(FD * 6 + (k % FS) * 2 + 1) * GT + ftid];
+    }
     const bool pml_here = PML && (tile_pml || (g.face[4].on && k <= g.face[4].hi2) || (g.face[5].on && k >= g.face[5].lo2));
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
@@ -1138,8 +1177,9 @@ struct StageLds {
   static constexpr int VW = 2 * VNL * TileGeom<BYT, RY, TX>::PL, VN = 2 * VNL * TileGeom<BYT, RY, TXN>::PL;
   static constexpr int SW = 2 * SNL * TileGeom<BYT, RY, TX>::PL, SN = 2 * SNL * TileGeom<BYT, RY, TXN>::PL;
   static constexpr int VOFF = (VW > VN ? VW : VN) + TX, FOFF = VOFF + (SW > SN ? SW : SN) + TX;
-  // mode C stress FIFO of the velocity group (one row per thread)
-  static constexpr int FF = (CURV && RY == 1) ? Fifo<1>::FD * 6 * TX * BYT : 0;
+  // mode C FIFOs of the velocity group (one row per thread): stresses, then
+  // the slopes handed to the stress group
+  static constexpr int FF = (CURV && RY == 1) ? (Fifo::FD * 6 + Fifo::FS * 2) * TX * BYT : 0;
   static constexpr int SIZE = FOFF + FF;
 };
 
@@ -1168,11 +1208,11 @@ __global__ __launch_bounds__(2 * TX * BYT) __attribute__((amdgpu_waves_per_eu(4)
       vel_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, sv, ss, wave, sh + L::FOFF);
   } else {
     if (it_ < g.p1)
-      str_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, ss, sv, wave - BYT);
+      str_body<XF, YF, ZF, FORM, BYT, RY, TXN, true, CURV>(g, it, slot, ss, sv, wave - BYT, sh + L::FOFF);
     else if (it_ < g.p2)
-      str_body<XF, YF, ZF, FORM, BYT, RY, TX, true, CURV>(g, it, slot, ss, sv, wave - BYT);
+      str_body<XF, YF, ZF, FORM, BYT, RY, TX, true, CURV>(g, it, slot, ss, sv, wave - BYT, sh + L::FOFF);
     else
-      str_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, ss, sv, wave - BYT);
+      str_body<XF, YF, ZF, FORM, BYT, RY, TX, false, CURV>(g, it, slot, ss, sv, wave - BYT, sh + L::FOFF);
   }
 }
 
