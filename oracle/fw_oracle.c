@@ -21,10 +21,9 @@
  *
  * Parity status of each part (what pins it) is stated at each function.
  * The coupled step is pinned by Stokes' point-force and moment-tensor
- * solutions (tests/test_oracle_analytic.py).  Parity unpinned: the
- * amplitude of the free-surface response to a surface force at the percent
- * level (a Lamb's-problem comparison did not converge); its one-stage
- * injection and the surface's Rayleigh speed and reciprocity are pinned.
+ * solutions (tests/test_oracle_analytic.py) and, for a surface force, by
+ * Lamb's problem with the buried share of the reference's smoothed source
+ * (tests/test_oracle_lamb.py); mode C by tests/test_oracle_topo.py.
  */
 #include <math.h>
 #include <stdlib.h>
