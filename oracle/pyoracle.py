@@ -13,12 +13,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "_build")
 
 
+# ORACLE_SANITIZE=1: load the AddressSanitizer / UndefinedBehaviorSanitizer
+# build (make san); the process must have libasan preloaded.
+SANITIZE = os.environ.get("ORACLE_SANITIZE") == "1"
+
+
 def build():
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-C", HERE] + (["san"] if SANITIZE else []), check=True)
 
 
 def _load(prec):
-    path = os.path.join(BUILD, f"liborc{prec}.so")
+    path = os.path.join(BUILD, f"liborc{prec}{'_san' if SANITIZE else ''}.so")
     if not os.path.exists(path):
         build()
     L = C.CDLL(path)
