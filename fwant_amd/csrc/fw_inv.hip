@@ -6,8 +6,12 @@
 // LSQR keeps A in CSR (the reference's row storage ra/ja/na) and A^T in CSR
 // built once on the host; every product is one wavefront per row with a
 // shuffle reduction, every dot product a fixed-order two-pass reduction, so
-// a solve is deterministic.  The scalar recurrences run on the host in
-// double, exactly as slsqr orders them.
+// a solve is deterministic.  The scalar recurrences (norms, the plane
+// rotation, the stopping tests) run on the device in one thread, in slsqr's
+// order with one rounding per operation, so that the host enqueues
+// iterations without waiting for them: it reads the device's stop flag once
+// per LSQR_POLL iterations, and every kernel of an iteration enqueued after
+// the stop returns at once.
 #include "../../include/fwant.h"
 
 #include <hip/hip_runtime.h>
@@ -33,6 +37,14 @@ int ifail(int code, const std::string& m) {
 
 constexpr int WAVE = 64;
 constexpr int RED_BLOCKS = 256;
+constexpr int LSQR_POLL = 16;
+
+// scalar state of a solve (slsqr's variables), in device memory
+struct LsqrState {
+  double alpha, beta, rhobar, phibar, rho, c, phi, theta, res2;
+  long itct;
+  int done;
+};
 
 // B[n] = sum over the pixels of block n of K(pixel) * volume(pixel)
 __global__ void k_assemble(const float* __restrict__ K, const long* __restrict__ ptr, const long* __restrict__ pix,
@@ -48,10 +60,11 @@ __global__ void k_assemble(const float* __restrict__ K, const long* __restrict__
 
 // y = A x, one wavefront per row
 __global__ void k_spmv(long nrow, const long* __restrict__ ptr, const long* __restrict__ col,
-                       const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y) {
+                       const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
+                       const LsqrState* __restrict__ S) {
   const long r = (long)blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
   const int lane = threadIdx.x % WAVE;
-  if (r >= nrow) return;
+  if (r >= nrow || S->done) return;
   double s = 0.0;
   for (long q = ptr[r] + lane; q < ptr[r + 1]; q += WAVE) s += val[q] * x[col[q]];
   for (int o = WAVE / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
@@ -60,7 +73,8 @@ __global__ void k_spmv(long nrow, const long* __restrict__ ptr, const long* __re
 
 // partial dot products (fixed grid), then one block sums the partials in order
 __global__ void k_dot_part(long n, const double* __restrict__ a, const double* __restrict__ b,
-                           double* __restrict__ part) {
+                           double* __restrict__ part, const LsqrState* __restrict__ S) {
+  if (S->done) return;
   __shared__ double sh[256];
   double s = 0.0;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) s += a[i] * b[i];
@@ -72,7 +86,8 @@ __global__ void k_dot_part(long n, const double* __restrict__ a, const double* _
   }
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
-__global__ void k_dot_final(const double* __restrict__ part, int np, double* __restrict__ out) {
+// the sum of the partials, in a fixed order, by one block
+__device__ double dot_final(const double* __restrict__ part, int np) {
   __shared__ double sh[256];
   double s = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
@@ -82,16 +97,70 @@ __global__ void k_dot_final(const double* __restrict__ part, int np, double* __r
     if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sh[0];
+  return sh[0];
 }
-// a = b - a*s (lupdat '-')
-__global__ void k_update(long n, double* __restrict__ a, const double* __restrict__ b, double s) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = b[i] - a[i] * s;
+// lnrliz's norm: S->beta (which 0) or S->alpha (which 1) = sqrt(a . a)
+__global__ void k_norm(const double* __restrict__ part, int np, LsqrState* __restrict__ S, int which) {
+  if (S->done) return;
+  const double s = dot_final(part, np);
+  if (threadIdx.x == 0) {
+    if (which) S->alpha = sqrt(s);
+    else S->beta = sqrt(s);
+  }
 }
-__global__ void k_scale(long n, double* __restrict__ a, double inv) {
+// lnrliz's division, unless the norm is (numerically) zero
+__global__ void k_scale(long n, double* __restrict__ a, const LsqrState* __restrict__ S, int which) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = a[i] / inv;
+  if (i >= n || S->done) return;
+  const double v0 = which ? S->alpha : S->beta;
+  if (fabs(v0) > 1.0e-20) a[i] = a[i] / v0;
+}
+// a = b - a*s (lupdat '-'), s = S->alpha (which 1) or S->beta (which 0)
+__global__ void k_update(long n, double* __restrict__ a, const double* __restrict__ b, const LsqrState* __restrict__ S,
+                         int which) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || S->done) return;
+  a[i] = b[i] - a[i] * (which ? S->alpha : S->beta);
+}
+__global__ void k_lsqr_start(LsqrState* __restrict__ S) {
+  S->phibar = S->beta;
+  S->rhobar = S->alpha;
+  S->res2 = 0.0;
+  S->itct = 0;
+}
+// slsqr's plane rotation (mod_LSQR.f90:224-233); rho = 0 ends the solve
+__global__ void k_lsqr_rotate(LsqrState* __restrict__ S) {
+#pragma clang fp contract(off)
+  if (S->done) return;
+  const double rhobar = S->rhobar, beta = S->beta, alpha = S->alpha;
+  const double rho = sqrt(rhobar * rhobar + beta * beta);
+  if (rho == 0.0) {
+    S->done = 1;
+    return;
+  }
+  const double c = rhobar / rho, s = beta / rho;
+  S->rho = rho;
+  S->c = c;
+  S->theta = s * alpha;
+  S->rhobar = -c * alpha;
+  S->phi = c * S->phibar;
+  S->phibar = s * S->phibar;
+}
+// the stopping tests after the solution update (mod_LSQR.f90:252-268),
+// from the partials of x . x
+__global__ void k_lsqr_test(const double* __restrict__ part, int np, LsqrState* __restrict__ S, double anorm,
+                            double atol, double btol, long maxit) {
+#pragma clang fp contract(off)
+  if (S->done) return;
+  const double xx = dot_final(part, np);
+  if (threadIdx.x != 0) return;
+  const double test1 = S->phibar / anorm / sqrt(xx);
+  const double test2 = S->alpha * fabs(S->c) / anorm;
+  S->itct += 1;
+  if (test1 <= atol || test2 <= btol || S->itct == maxit) {
+    S->res2 = S->phibar;
+    S->done = 1;
+  }
 }
 __global__ void k_copy(long n, double* __restrict__ a, const double* __restrict__ b) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -103,12 +172,12 @@ __global__ void k_fill0(long n, double* __restrict__ a) {
 }
 // x += phi*w/rho; w = v - theta*w/rho; sig += (w/rho)^2  (slsqr's solution update)
 __global__ void k_xw(long n, double* __restrict__ x, double* __restrict__ w, const double* __restrict__ v,
-                     double* __restrict__ sig, double rho, double phi, double theta) {
+                     double* __restrict__ sig, const LsqrState* __restrict__ S) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double t = w[i] / rho;
-  x[i] = x[i] + phi * t;
-  w[i] = v[i] - theta * t;
+  if (i >= n || S->done) return;
+  const double t = w[i] / S->rho;
+  x[i] = x[i] + S->phi * t;
+  w[i] = v[i] - S->theta * t;
   sig[i] = sig[i] + t * t;
 }
 
@@ -121,28 +190,22 @@ struct fwl_s {
   long *ptr = nullptr, *col = nullptr, *tptr = nullptr, *tcol = nullptr;
   double *val = nullptr, *tval = nullptr;
   double *b = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *x = nullptr, *q = nullptr, *sig = nullptr;
-  double *part = nullptr, *scal = nullptr;
+  double* part = nullptr;
+  LsqrState* st = nullptr;
   double anorm = 0.0;
   hipStream_t stream = nullptr;
 };
 
-static double ddot(fwl_handle h, long n, const double* a, const double* b) {
-  hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, h->stream, n, a, b, h->part);
-  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, h->stream, h->part, RED_BLOCKS, h->scal);
-  double r = 0.0;
-  hipMemcpyAsync(&r, h->scal, sizeof(double), hipMemcpyDeviceToHost, h->stream);
-  hipStreamSynchronize(h->stream);
-  return r;
-}
-// lnrliz: norm, then divide unless the norm is (numerically) zero
-static double normalize(fwl_handle h, long n, double* a) {
-  const double v0 = std::sqrt(ddot(h, n, a, a));
-  if (std::fabs(v0) > 1.0e-20) hipLaunchKernelGGL(k_scale, dim3(nb(n, 256)), dim3(256), 0, h->stream, n, a, v0);
-  return v0;
+// lnrliz: the norm into S->beta (which 0) or S->alpha (which 1), then the
+// division unless the norm is (numerically) zero
+static void normalize(fwl_handle h, long n, double* a, int which) {
+  hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, h->stream, n, a, a, h->part, h->st);
+  hipLaunchKernelGGL(k_norm, dim3(1), dim3(256), 0, h->stream, h->part, RED_BLOCKS, h->st, which);
+  hipLaunchKernelGGL(k_scale, dim3(nb(n, 256)), dim3(256), 0, h->stream, n, a, h->st, which);
 }
 static void spmv(fwl_handle h, long nrow, const long* ptr, const long* col, const double* val, const double* x,
                  double* y) {
-  hipLaunchKernelGGL(k_spmv, dim3(nb(nrow, 256 / WAVE)), dim3(256), 0, h->stream, nrow, ptr, col, val, x, y);
+  hipLaunchKernelGGL(k_spmv, dim3(nb(nrow, 256 / WAVE)), dim3(256), 0, h->stream, nrow, ptr, col, val, x, y, h->st);
 }
 
 extern "C" {
@@ -168,7 +231,7 @@ int fwl_destroy(fwl_handle h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   for (void* p : {(void*)h->ptr, (void*)h->col, (void*)h->tptr, (void*)h->tcol, (void*)h->val, (void*)h->tval,
                   (void*)h->b, (void*)h->u, (void*)h->v, (void*)h->w, (void*)h->x, (void*)h->q, (void*)h->sig,
-                  (void*)h->part, (void*)h->scal})
+                  (void*)h->part, (void*)h->st})
     hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
   delete h;
@@ -211,7 +274,7 @@ int fwl_create(long nrow, long ncol, long nnz, const long* row_ptr, const long* 
       hipMalloc(&h->q, sizeof(double) * std::max(nrow, ncol)) || hipMalloc(&h->v, sizeof(double) * ncol) ||
       hipMalloc(&h->w, sizeof(double) * ncol) || hipMalloc(&h->x, sizeof(double) * ncol) ||
       hipMalloc(&h->sig, sizeof(double) * ncol) || hipMalloc(&h->part, sizeof(double) * RED_BLOCKS) ||
-      hipMalloc(&h->scal, sizeof(double)))
+      hipMalloc(&h->st, sizeof(LsqrState)))
     return fail("fwl_create: hipMalloc failed");
   if (hipMemcpy(h->ptr, row_ptr, sizeof(long) * (nrow + 1), hipMemcpyHostToDevice) ||
       (nnz && hipMemcpy(h->col, col, sizeof(long) * nnz, hipMemcpyHostToDevice)) ||
@@ -231,39 +294,40 @@ int fwl_solve(fwl_handle h, double atol, double btol, long maxit, double* x_out,
   if (!h || !x_out || maxit <= 0) return ifail(FW_ERR_ARG, "fwl_solve: bad argument");
   const long m = h->nrow, n = h->ncol;
   hipStream_t st = h->stream;
+  LsqrState s0 = {};
+  ICHK(hipMemcpyAsync(h->st, &s0, sizeof(LsqrState), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_copy, dim3(nb(m, 256)), dim3(256), 0, st, m, h->u, h->b);
-  double beta = normalize(h, m, h->u);
+  normalize(h, m, h->u, 0);                           // beta u = b
   spmv(h, n, h->tptr, h->tcol, h->tval, h->u, h->v);  // v = A^T u
-  double alpha = normalize(h, n, h->v);
+  normalize(h, n, h->v, 1);                           // alpha v = A^T u
   hipLaunchKernelGGL(k_copy, dim3(nb(n, 256)), dim3(256), 0, st, n, h->w, h->v);
   hipLaunchKernelGGL(k_fill0, dim3(nb(n, 256)), dim3(256), 0, st, n, h->x);
   hipLaunchKernelGGL(k_fill0, dim3(nb(n, 256)), dim3(256), 0, st, n, h->sig);
-  double phibar = beta, rhobar = alpha, res2 = 0.0;
-  long itct = 0;
-  for (;;) {
-    spmv(h, m, h->ptr, h->col, h->val, h->v, h->q);  // q = A v
-    hipLaunchKernelGGL(k_update, dim3(nb(m, 256)), dim3(256), 0, st, m, h->u, h->q, alpha);
-    beta = normalize(h, m, h->u);
-    spmv(h, n, h->tptr, h->tcol, h->tval, h->u, h->q);  // q = A^T u
-    hipLaunchKernelGGL(k_update, dim3(nb(n, 256)), dim3(256), 0, st, n, h->v, h->q, beta);
-    alpha = normalize(h, n, h->v);
-    const double rho = std::sqrt(rhobar * rhobar + beta * beta);
-    if (rho == 0.0) break;
-    const double c = rhobar / rho, s = beta / rho, theta = s * alpha;
-    rhobar = -c * alpha;
-    const double phi = c * phibar;
-    phibar = s * phibar;
-    hipLaunchKernelGGL(k_xw, dim3(nb(n, 256)), dim3(256), 0, st, n, h->x, h->w, h->v, h->sig, rho, phi, theta);
-    const double rnorm = phibar;
-    const double test1 = rnorm / h->anorm / std::sqrt(ddot(h, n, h->x, h->x));
-    const double test2 = alpha * std::fabs(c) / h->anorm;
-    itct++;
-    if (test1 <= atol || test2 <= btol || itct == maxit) {
-      res2 = phibar;
-      break;
+  hipLaunchKernelGGL(k_lsqr_start, dim3(1), dim3(1), 0, st, h->st);
+  // iterations are enqueued LSQR_POLL at a time; the device sets done at
+  // the stopping test (at the latest at itct == maxit)
+  LsqrState sh = {};
+  for (long it = 0; it < maxit; it += LSQR_POLL) {
+    for (long j = it; j < std::min(maxit, it + LSQR_POLL); ++j) {
+      spmv(h, m, h->ptr, h->col, h->val, h->v, h->q);  // q = A v
+      hipLaunchKernelGGL(k_update, dim3(nb(m, 256)), dim3(256), 0, st, m, h->u, h->q, h->st, 1);
+      normalize(h, m, h->u, 0);                           // beta u = A v - alpha u
+      spmv(h, n, h->tptr, h->tcol, h->tval, h->u, h->q);  // q = A^T u
+      hipLaunchKernelGGL(k_update, dim3(nb(n, 256)), dim3(256), 0, st, n, h->v, h->q, h->st, 0);
+      normalize(h, n, h->v, 1);                           // alpha v = A^T u - beta v
+      hipLaunchKernelGGL(k_lsqr_rotate, dim3(1), dim3(1), 0, st, h->st);
+      hipLaunchKernelGGL(k_xw, dim3(nb(n, 256)), dim3(256), 0, st, n, h->x, h->w, h->v, h->sig, h->st);
+      hipLaunchKernelGGL(k_dot_part, dim3(RED_BLOCKS), dim3(256), 0, st, n, h->x, h->x, h->part, h->st);
+      hipLaunchKernelGGL(k_lsqr_test, dim3(1), dim3(256), 0, st, h->part, RED_BLOCKS, h->st, h->anorm, atol, btol,
+                         maxit);
     }
+    ICHK(hipGetLastError());
+    ICHK(hipMemcpyAsync(&sh, h->st, sizeof(LsqrState), hipMemcpyDeviceToHost, st));
+    ICHK(hipStreamSynchronize(st));
+    if (sh.done) break;
   }
-  ICHK(hipStreamSynchronize(st));
+  const long itct = sh.itct;
+  const double res2 = sh.res2;
   ICHK(hipMemcpy(x_out, h->x, sizeof(double) * n, hipMemcpyDeviceToHost));
   if (err_out) {
     std::vector<double> sig(n);
