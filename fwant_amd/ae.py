@@ -60,7 +60,7 @@ class AE:
     """Weights and Adam state in HBM; forward / backward / step on batches.
     layers: the network (default LAYERS), n_enc: its encoder layers."""
 
-    def __init__(self, weights=None, seed=0, nchunk=64, layers=LAYERS, n_enc=N_ENC):
+    def __init__(self, weights=None, seed=0, nchunk=256, layers=LAYERS, n_enc=N_ENC):
         self.L = _lib.lib()
         self.layers, self.n_enc = tuple(layers), n_enc
         _, _, co, _, do, _, _, _, _ = self.layers[n_enc - 1]

@@ -1,6 +1,7 @@
 """The SGT block autoencoder on the GPU:
-  - convolution forward / backward-data / backward-weight kernels equal the
-    numpy oracle (FP64) to FP32 accuracy, for both layer shapes used;
+  - convolution forward / backward-data / backward-weight kernels (implicit
+    GEMMs on the matrix cores) equal the numpy oracle (FP64) to FP32
+    accuracy, for both layer kinds used and the edge cases of their tiling;
   - the whole network's loss and gradients (BF16-rounded activations) equal
     the oracle's;
   - trained on SGT blocks the solver produces, the network reconstructs
@@ -21,12 +22,18 @@ def rel(a, b):
     return np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("K,S,P,Di", [(3, 1, 1, 8), (4, 2, 1, 8)])
-def test_conv_primitives(gpu_lib, K, S, P, Di):
+# (K, S, P, Di, N, Ci, Co): the network's two layer kinds at small and full
+# size, more output channels than one 16-row matrix tile, an odd volume with
+# stride 2 and no padding; the three launch shapes (waves splitting the
+# reduction of few columns, 16-column and 64-column wave tiles)
+@pytest.mark.parametrize("K,S,P,Di,N,Ci,Co", [(3, 1, 1, 8, 3, 5, 7), (4, 2, 1, 8, 3, 5, 7), (4, 2, 1, 16, 3, 16, 16),
+                                             (3, 1, 1, 6, 2, 6, 20), (3, 2, 0, 7, 2, 4, 3), (3, 1, 1, 16, 16, 16, 6),
+                                             (3, 1, 1, 16, 64, 6, 16)])
+def test_conv_primitives(gpu_lib, K, S, P, Di, N, Ci, Co):
     from fwant_amd._lib import DeviceArray, lib
     from oracle import ae_ref
-    rng = np.random.default_rng(K + Di)
-    N, Ci, Co = 3, 5, 7
+    rng = np.random.default_rng(K + Di + N)
+    print(f"conv K={K} S={S} P={P} Di={Di} N={N} Ci={Ci} Co={Co}", flush=True)
     Do = (Di + 2 * P - K) // S + 1
     x = rng.standard_normal((N, Ci, Di, Di, Di)).astype(np.float32)
     w = rng.standard_normal((Co, Ci, K ** 3)).astype(np.float32)
@@ -46,7 +53,13 @@ def test_conv_primitives(gpu_lib, K, S, P, Di):
     assert rel(dx.to_host(), ae_ref.conv_bwd_data(dy.astype(np.float64), W, S, P, Di)) < 1e-6
     gw, gb = ae_ref.conv_bwd_weight(x.astype(np.float64), dy.astype(np.float64), S, P, K)
     h = dwb.to_host()
-    assert rel(h[:-Co], gw.ravel()) < 1e-6 and rel(h[-Co:], gb) < 1e-6
+    # each of the 16 partial sums adds N Do^3 / 16 products in FP32: the
+    # rounding error of such a sum grows like the square root of its length
+    terms = N * Do ** 3 / 16
+    tol = 1e-6 * max(1.0, np.sqrt(terms / 1024))
+    ew, eb = rel(h[:-Co], gw.ravel()), rel(h[-Co:], gb)
+    print(f"dW rel {ew:.2e}, db rel {eb:.2e} ({terms:.0f} terms per partial sum, bound {tol:.1e})")
+    assert ew < tol and eb < tol
 
 
 def test_network_gradients(gpu_lib):
