@@ -183,37 +183,36 @@ __device__ __forceinline__ float* aux_base(const FaceArgs& f, int slot) {
 struct AuxVals {
   float p[3], a[3], t[3];
   float b1, ad, dd;
-  int face;  // 0..5, or -1: the cell is in no slab
+  int face;   // 0..5, or -1: the cell is in no slab
+  float* bp;  // set A, first of the three variables, this cell
+  int cnt;    // the face's cell count
 };
 __device__ __forceinline__ float sel3(int m, float v0, float v1, float v2) { return m == 0 ? v0 : (m == 1 ? v1 : v2); }
 
-// auxiliary RK update of variable v in the RK form above (the reference's
+// auxiliary RK update of variable m (0..2) of the three at bp (set A, this
+// cell) of a face with cnt cells, in the RK form above (the reference's
 // abs_RK_beg/inn/fin, mod_abs_npml.F90:738-1105, in 4-register form); psi is
 // the stage input's value (W_1 / W_3 at forms 1 / 3).  pre: the set-A /
 // set-T values were loaded ahead (pa, pt)
 template <int FORM, bool FULL>
-__device__ __forceinline__ void aux_rk(const FaceArgs& f, const StageArgs& g, int slot, long li, int v, float psi, float dpsi,
-                                       bool pre, float pa, float pt) {
-  const long S = 6 * f.cnt, q = v * f.cnt + li;
-  float* const d = aux_base(f, slot);
+__device__ __forceinline__ void aux_rk_at(const StageArgs& g, float* bp, long cnt, int m, float psi, float dpsi, bool pre,
+                                          float pa, float pt) {
+  const long S = 6 * cnt, q = m * cnt;
   if constexpr (FORM == 0) {
-    d[g.setout * S + q] = psi + g.a * dpsi;
+    bp[g.setout * S + q] = psi + g.a * dpsi;
   } else {
     if constexpr (FORM == 1) {
-      const float A = FULL && pre ? pa : d[q];
-      d[g.setout * S + q] = A + g.a * dpsi;
-      d[3 * S + q] = g.c * A + g.r * psi + g.b * dpsi;
+      const float A = FULL && pre ? pa : bp[q];
+      bp[g.setout * S + q] = A + g.a * dpsi;
+      bp[3 * S + q] = g.c * A + g.r * psi + g.b * dpsi;
     } else if constexpr (FORM == 2) {
-      const float A = FULL && pre ? pa : d[q];
-      d[g.setout * S + q] = A + g.a * dpsi;
+      const float A = FULL && pre ? pa : bp[q];
+      bp[g.setout * S + q] = A + g.a * dpsi;
     } else {
-      const float T = FULL && pre ? pt : d[3 * S + q];
-      d[g.setout * S + q] = T + g.r * psi + g.b * dpsi;
+      const float T = FULL && pre ? pt : bp[3 * S + q];
+      bp[g.setout * S + q] = T + g.r * psi + g.b * dpsi;
     }
   }
-}
-__device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, int slot, long li, int v) {
-  return aux_base(f, slot)[g.setin * 6 * f.cnt + v * f.cnt + li];
 }
 
 // Prefetch of the auxiliary values of the first face slab holding a cell,
@@ -224,10 +223,11 @@ __device__ __forceinline__ float aux_in(const FaceArgs& f, const StageArgs& g, i
 // before the plane's barrier; cells in more than one slab (edges, corners)
 // load the other slabs' values where they are applied.
 struct AuxPre {
-  const float* base;  // set A, first of the three variables, this cell, plane k0
+  float* base;        // set A, first of the three variables, this cell, plane k0
   int face;           // 0..3, or -1: no x or y slab
   int cnt, kst;       // face cell count, index step per plane
-  int mask;           // bit n: the cell is in x or y slab n (0..3)
+  int mask;           // bit n: the cell is in x or y slab n (0..3); bits 8, 9:
+                      // the column is in the (i, j) extent of z slab 4, 5
 };
 __device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int slot, int i, int j, int k0, int v0, bool comp) {
   AuxPre r{nullptr, -1, 0, 0, 0};
@@ -243,37 +243,47 @@ __device__ __forceinline__ AuxPre aux_pre(const StageArgs& g, int slot, int i, i
       }
     }
   }
+  for (int n = 4; n < 6; ++n) {
+    const FaceArgs& f = g.face[n];
+    if (comp && f.on && i >= f.lo0 && i <= f.hi0 && j >= f.lo1 && j <= f.hi1) r.mask |= 1 << (n + 4);
+  }
   return r;
+}
+// plane k lies in z slab n (4, 5): uniform over the block
+__device__ __forceinline__ bool zplane(const StageArgs& g, int n, int k) {
+  return g.face[n].on && k >= g.face[n].lo2 && k <= g.face[n].hi2;
 }
 // slabs holding cell (i, j, k) of a computed row: the column's x / y slabs
 // and the plane's z slabs
 __device__ __forceinline__ int face_mask(const StageArgs& g, const AuxPre& a, int i, int j, int k) {
-  return a.mask | (in_face(g.face[4], i, j, k) ? 16 : 0) | (in_face(g.face[5], i, j, k) ? 32 : 0);
+  return (a.mask & 15) | ((a.mask & 256) && zplane(g, 4, k) ? 16 : 0) | ((a.mask & 512) && zplane(g, 5, k) ? 32 : 0);
 }
 template <int FORM, bool FULL>
 __device__ __forceinline__ void aux_plane(const StageArgs& g, const AuxPre& a, int slot, int i, int j, int k, int k0, int v0,
                                           bool comp, AuxVals& r) {
-  const float* b = nullptr;
+  float* b = nullptr;
   long c = 0;
   int ia = 0;
-  r = AuxVals{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f, 0.f, -1};
+  r = AuxVals{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, 0.f, 0.f, 0.f, -1, nullptr, 0};
   if (a.face >= 0) {
     r.face = a.face;
     b = a.base + (long)(k - k0) * a.kst;
     c = a.cnt;
     ia = a.face < 2 ? i : g.nx + j;
-  } else if (comp && in_face(g.face[4], i, j, k)) {
+  } else if ((a.mask & 256) && zplane(g, 4, k)) {
     r.face = 4;
     c = g.face[4].cnt;
     b = aux_base(g.face[4], slot) + (long)v0 * c + face_index(g.face[4], i, j, k);
     ia = g.nx + g.ny + k;
-  } else if (comp && in_face(g.face[5], i, j, k)) {
+  } else if ((a.mask & 512) && zplane(g, 5, k)) {
     r.face = 5;
     c = g.face[5].cnt;
     b = aux_base(g.face[5], slot) + (long)v0 * c + face_index(g.face[5], i, j, k);
     ia = g.nx + g.ny + k;
   }
   if (r.face >= 0) {
+    r.bp = b;
+    r.cnt = (int)c;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       r.p[m] = b[g.setin * 6 * c + m * c];
@@ -289,113 +299,154 @@ __device__ __forceinline__ void aux_plane(const StageArgs& g, const AuxPre& a, i
 // psi only
 constexpr bool AUX_FULL_VEL = true, AUX_FULL_STR = true;
 
+// One face slab's corrections of hV (ax: its axis) and the RK update of its
+// three auxiliaries at bp (variables 3..5 of the face; pre: their set-A /
+// set-T values were loaded ahead, in pv)
+template <int FORM, bool FULL>
+__device__ __forceinline__ void vel_face(const StageArgs& g, int ax, float b1, float ad, float Dd, float p3, float p4,
+                                         float p5, float* bp, long cnt, bool pre, const AuxVals& pv, float h[3], float rr,
+                                         float DxTxx, float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz,
+                                         float DzTzz, float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz,
+                                         float Txz, float Tyz, float rzk, float rxs, float xc) {
+  const float b1_1 = b1 - 1.0f;
+  float D0, D1, D2;  // corrected derivatives feeding hVx, hVy, hVz
+  float q0, q1, q2;  // their psi
+  int v0, v1, v2;
+  if (ax == 0) {
+    D0 = (DxTxx + 2.0f * Txz) * rzk; q0 = p3; v0 = 3;
+    D1 = (DxTxy + Tyz) * rzk; q1 = p4; v1 = 4;
+    D2 = (DxTxz - Txx + Tzz) * rzk; q2 = p5; v2 = 5;
+  } else if (ax == 1) {
+    D0 = (DyTxy * rxs + (Txx - Tyy) * xc + Txz) * rzk; q0 = p4; v0 = 4;
+    D1 = (DyTyy * rxs + 2.0f * Txy * xc + 2.0f * Tyz) * rzk; q1 = p3; v1 = 3;
+    D2 = (DyTyz * rxs + Txz * xc + (Tzz - Tyy)) * rzk; q2 = p5; v2 = 5;
+  } else {
+    D0 = DzTxz; q0 = p4; v0 = 4;
+    D1 = DzTyz; q1 = p5; v1 = 5;
+    D2 = DzTzz; q2 = p3; v2 = 3;
+  }
+  h[0] += (b1_1 * D0 - b1 * q0) * rr;
+  h[1] += (b1_1 * D1 - b1 * q1) * rr;
+  h[2] += (b1_1 * D2 - b1 * q2) * rr;
+  aux_rk_at<FORM, FULL>(g, bp, cnt, v0 - 3, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]),
+                        sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
+  aux_rk_at<FORM, FULL>(g, bp, cnt, v1 - 3, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]),
+                        sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
+  aux_rk_at<FORM, FULL>(g, bp, cnt, v2 - 3, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]),
+                        sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
+}
+
 // CFS-PML corrections of hV on the face slabs containing (i, j, k), in the
 // reference's face order x1 x2 y1 y2 z1 z2 (mod_abs_npml.F90:1149-2123, the
 // stress-derivative half).  D* are the raw one-sided derivatives times the
-// 1-D metric (before fluid zeroing).
+// 1-D metric (before fluid zeroing).  The first slab holding the cell (pf)
+// takes the values loaded with the plane (pv); the faces after it (cells at
+// slab edges and corners only) load theirs here.
 template <int FORM, bool FULL>
 __device__ __forceinline__ void pml_vel(const StageArgs& g, int slot, int i, int j, int k, float h[3], float rr, float DxTxx,
                                         float DxTxy, float DxTxz, float DyTyy, float DyTxy, float DyTyz, float DzTzz,
                                         float DzTxz, float DzTyz, float Txx, float Tyy, float Txy, float Tzz, float Txz,
                                         float Tyz, float rzk, float rxs, float xc, int mask, int pf, const AuxVals pv) {
+  if (pf >= 0)
+    vel_face<FORM, FULL>(g, pf >> 1, pv.b1, pv.ad, pv.dd, pv.p[0], pv.p[1], pv.p[2], pv.bp, pv.cnt, true, pv, h, rr,
+                         DxTxx, DxTxy, DxTxz, DyTyy, DyTxy, DyTyz, DzTzz, DzTxz, DzTyz, Txx, Tyy, Txy, Tzz, Txz, Tyz, rzk,
+                         rxs, xc);
+  const int rest = pf >= 0 ? mask & ~(1 << pf) : mask;
+  if (rest == 0) return;
 #pragma nounroll
   for (int n = 0; n < 6; ++n) {
+    if (!((rest >> n) & 1)) continue;
     const FaceArgs& f = g.face[n];
-    if (!((mask >> n) & 1)) continue;
     const int ax = n >> 1;
     const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
-    const bool pre = n == pf;
-    const float b1 = pre ? pv.b1 : g.pb1[ia], b1_1 = b1 - 1.0f, ad = pre ? pv.ad : g.pad[ia], Dd = pre ? pv.dd : g.pdd[ia];
-    const long li = face_index(f, i, j, k);
-    const float p3 = pre ? pv.p[0] : aux_in(f, g, slot, li, 3), p4 = pre ? pv.p[1] : aux_in(f, g, slot, li, 4),
-                p5 = pre ? pv.p[2] : aux_in(f, g, slot, li, 5);
-    float D0, D1, D2;  // corrected derivatives feeding hVx, hVy, hVz
-    float q0, q1, q2;  // their psi
-    int v0, v1, v2;
-    if (ax == 0) {
-      D0 = (DxTxx + 2.0f * Txz) * rzk; q0 = p3; v0 = 3;
-      D1 = (DxTxy + Tyz) * rzk; q1 = p4; v1 = 4;
-      D2 = (DxTxz - Txx + Tzz) * rzk; q2 = p5; v2 = 5;
-    } else if (ax == 1) {
-      D0 = (DyTxy * rxs + (Txx - Tyy) * xc + Txz) * rzk; q0 = p4; v0 = 4;
-      D1 = (DyTyy * rxs + 2.0f * Txy * xc + 2.0f * Tyz) * rzk; q1 = p3; v1 = 3;
-      D2 = (DyTyz * rxs + Txz * xc + (Tzz - Tyy)) * rzk; q2 = p5; v2 = 5;
-    } else {
-      D0 = DzTxz; q0 = p4; v0 = 4;
-      D1 = DzTyz; q1 = p5; v1 = 5;
-      D2 = DzTzz; q2 = p3; v2 = 3;
-    }
-    h[0] += (b1_1 * D0 - b1 * q0) * rr;
-    h[1] += (b1_1 * D1 - b1 * q1) * rr;
-    h[2] += (b1_1 * D2 - b1 * q2) * rr;
-    aux_rk<FORM, FULL>(f, g, slot, li, v0, q0, Dd * D0 - ad * q0, pre, sel3(v0 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v0 - 3, pv.t[0], pv.t[1], pv.t[2]));
-    aux_rk<FORM, FULL>(f, g, slot, li, v1, q1, Dd * D1 - ad * q1, pre, sel3(v1 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v1 - 3, pv.t[0], pv.t[1], pv.t[2]));
-    aux_rk<FORM, FULL>(f, g, slot, li, v2, q2, Dd * D2 - ad * q2, pre, sel3(v2 - 3, pv.a[0], pv.a[1], pv.a[2]), sel3(v2 - 3, pv.t[0], pv.t[1], pv.t[2]));
+    const long cnt = f.cnt;
+    float* const bp = aux_base(f, slot) + 3 * cnt + face_index(f, i, j, k);
+    const long in = g.setin * 6 * cnt;
+    vel_face<FORM, FULL>(g, ax, g.pb1[ia], g.pad[ia], g.pdd[ia], bp[in], bp[in + cnt], bp[in + 2 * cnt], bp, cnt, false,
+                         pv, h, rr, DxTxx, DxTxy, DxTxz, DyTyy, DyTxy, DyTyz, DzTzz, DzTxz, DzTyz, Txx, Tyy, Txy, Tzz, Txz,
+                         Tyz, rzk, rxs, xc);
   }
 }
 
-// CFS-PML corrections of hT (velocity-derivative half), including the
-// free-surface Hooke fix-ups of the x and y faces (mod_abs_npml.F90:1303-1333).
+// One face slab's corrections of hT (ax: its axis; surf: the free-surface
+// Hooke fix-ups of the x and y faces, mod_abs_npml.F90:1303-1333) and the RK
+// update of its three auxiliaries at bp (variables 0..2 of the face; pre:
+// their set-A / set-T values were loaded ahead, in pv).
 // h: Txx Tyy Tzz Txy Txz Tyz.  DzV* are the standard 5-tap z derivatives.
+template <int FORM, bool FULL>
+__device__ __forceinline__ void str_face(const StageArgs& g, int ax, bool surf, float b1, float ad, float Dd, float p0,
+                                         float p1, float p2, float* bp, long cnt, bool pre, const AuxVals& pv, float h[6],
+                                         float lam, float miu, float DxVx, float DxVy, float DxVz, float DyVx, float DyVy,
+                                         float DyVz, float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz,
+                                         float rzk, float rxs, float xc) {
+  const float tc13 = lam, tc66 = miu, tc11 = tc13 + 2.0f * tc66;
+  const float tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11, tc44 = tc66, tc55 = tc66;
+  const float b1_1 = b1 - 1.0f;
+  float D0, D1, D2;
+  if (ax == 0) {
+    D0 = (DxVx + Vz) * rzk; D1 = DxVy * rzk; D2 = (DxVz - Vx) * rzk;
+    h[0] += b1_1 * tc11 * D0 - b1 * tc11 * p0;
+    h[1] += b1_1 * tc12 * D0 - b1 * tc12 * p0;
+    h[2] += b1_1 * tc13 * D0 - b1 * tc13 * p0;
+    h[3] += b1_1 * tc66 * D1 - b1 * tc66 * p1;
+    h[4] += b1_1 * tc55 * D2 - b1 * tc55 * p2;
+    if (surf) {
+      h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
+      h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
+    }
+  } else if (ax == 1) {
+    D0 = (DyVx * rxs - Vy * xc) * rzk;
+    D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
+    D2 = (DyVz * rxs - Vy) * rzk;
+    h[3] += b1_1 * tc66 * D0 - b1 * tc66 * p0;
+    h[0] += b1_1 * tc12 * D1 - b1 * tc12 * p1;
+    h[1] += b1_1 * tc22 * D1 - b1 * tc22 * p1;
+    h[2] += b1_1 * tc23 * D1 - b1 * tc23 * p1;
+    h[5] += b1_1 * tc44 * D2 - b1 * tc44 * p2;
+    if (surf) {
+      h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
+      h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
+    }
+  } else {
+    D0 = DzVx; D1 = DzVy; D2 = DzVz;
+    h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
+    h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
+    h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
+    h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
+    h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
+  }
+  aux_rk_at<FORM, FULL>(g, bp, cnt, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
+  aux_rk_at<FORM, FULL>(g, bp, cnt, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
+  aux_rk_at<FORM, FULL>(g, bp, cnt, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
+}
+
+// CFS-PML corrections of hT on the face slabs containing (i, j, k), in the
+// reference's face order; the first slab holding the cell (pf) takes the
+// values loaded with the plane (pv), the faces after it (slab edges and
+// corners only) load theirs here.
 template <int FORM, bool FULL>
 __device__ __forceinline__ void pml_str(const StageArgs& g, int slot, int i, int j, int k, float h[6], float lam, float miu,
                                         float DxVx, float DxVy, float DxVz, float DyVx, float DyVy, float DyVz,
                                         float DzVx, float DzVy, float DzVz, float Vx, float Vy, float Vz, float rzk,
                                         float rxs, float xc, int mask, int pf, const AuxVals pv) {
-  const float tc13 = lam, tc66 = miu, tc11 = tc13 + 2.0f * tc66;
-  const float tc12 = tc13, tc22 = tc11, tc23 = tc13, tc33 = tc11, tc44 = tc66, tc55 = tc66;
   const bool surf = g.freenode && k == g.nk2;
+  if (pf >= 0)
+    str_face<FORM, FULL>(g, pf >> 1, surf, pv.b1, pv.ad, pv.dd, pv.p[0], pv.p[1], pv.p[2], pv.bp, pv.cnt, true, pv, h,
+                         lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, DzVx, DzVy, DzVz, Vx, Vy, Vz, rzk, rxs, xc);
+  const int rest = pf >= 0 ? mask & ~(1 << pf) : mask;
+  if (rest == 0) return;
 #pragma nounroll
   for (int n = 0; n < 6; ++n) {
+    if (!((rest >> n) & 1)) continue;
     const FaceArgs& f = g.face[n];
-    if (!((mask >> n) & 1)) continue;
     const int ax = n >> 1;
     const int ia = ax == 0 ? i : (ax == 1 ? g.nx + j : g.nx + g.ny + k);
-    const bool pre = n == pf;
-    const float b1 = pre ? pv.b1 : g.pb1[ia], b1_1 = b1 - 1.0f, ad = pre ? pv.ad : g.pad[ia], Dd = pre ? pv.dd : g.pdd[ia];
-    const long li = face_index(f, i, j, k);
-    const float p0 = pre ? pv.p[0] : aux_in(f, g, slot, li, 0), p1 = pre ? pv.p[1] : aux_in(f, g, slot, li, 1),
-                p2 = pre ? pv.p[2] : aux_in(f, g, slot, li, 2);
-    if (ax == 0) {
-      const float D0 = (DxVx + Vz) * rzk, D1 = DxVy * rzk, D2 = (DxVz - Vx) * rzk;
-      h[0] += b1_1 * tc11 * D0 - b1 * tc11 * p0;
-      h[1] += b1_1 * tc12 * D0 - b1 * tc12 * p0;
-      h[2] += b1_1 * tc13 * D0 - b1 * tc13 * p0;
-      h[3] += b1_1 * tc66 * D1 - b1 * tc66 * p1;
-      h[4] += b1_1 * tc55 * D2 - b1 * tc55 * p2;
-      if (surf) {
-        h[0] += -b1_1 * tc13 * tc13 / tc33 * D0 + b1 * tc13 * tc13 / tc33 * p0;
-        h[1] += -b1_1 * tc23 * tc13 / tc33 * D0 + b1 * tc23 * tc13 / tc33 * p0;
-      }
-      aux_rk<FORM, FULL>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<FORM, FULL>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<FORM, FULL>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
-    } else if (ax == 1) {
-      const float D0 = (DyVx * rxs - Vy * xc) * rzk;
-      const float D1 = (DyVy * rxs + Vx * xc + Vz) * rzk;
-      const float D2 = (DyVz * rxs - Vy) * rzk;
-      h[3] += b1_1 * tc66 * D0 - b1 * tc66 * p0;
-      h[0] += b1_1 * tc12 * D1 - b1 * tc12 * p1;
-      h[1] += b1_1 * tc22 * D1 - b1 * tc22 * p1;
-      h[2] += b1_1 * tc23 * D1 - b1 * tc23 * p1;
-      h[5] += b1_1 * tc44 * D2 - b1 * tc44 * p2;
-      if (surf) {
-        h[0] += -b1_1 * tc13 * tc23 / tc33 * D1 + b1 * tc13 * tc23 / tc33 * p1;
-        h[1] += -b1_1 * tc23 * tc23 / tc33 * D1 + b1 * tc23 * tc23 / tc33 * p1;
-      }
-      aux_rk<FORM, FULL>(f, g, slot, li, 0, p0, Dd * D0 - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<FORM, FULL>(f, g, slot, li, 1, p1, Dd * D1 - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<FORM, FULL>(f, g, slot, li, 2, p2, Dd * D2 - ad * p2, pre, pv.a[2], pv.t[2]);
-    } else {
-      h[4] += b1_1 * tc55 * DzVx - b1 * tc55 * p0;
-      h[5] += b1_1 * tc44 * DzVy - b1 * tc44 * p1;
-      h[0] += b1_1 * tc13 * DzVz - b1 * tc13 * p2;
-      h[1] += b1_1 * tc23 * DzVz - b1 * tc23 * p2;
-      h[2] += b1_1 * tc33 * DzVz - b1 * tc33 * p2;
-      aux_rk<FORM, FULL>(f, g, slot, li, 0, p0, Dd * DzVx - ad * p0, pre, pv.a[0], pv.t[0]);
-      aux_rk<FORM, FULL>(f, g, slot, li, 1, p1, Dd * DzVy - ad * p1, pre, pv.a[1], pv.t[1]);
-      aux_rk<FORM, FULL>(f, g, slot, li, 2, p2, Dd * DzVz - ad * p2, pre, pv.a[2], pv.t[2]);
-    }
+    const long cnt = f.cnt;
+    float* const bp = aux_base(f, slot) + face_index(f, i, j, k);
+    const long in = g.setin * 6 * cnt;
+    str_face<FORM, FULL>(g, ax, surf, g.pb1[ia], g.pad[ia], g.pdd[ia], bp[in], bp[in + cnt], bp[in + 2 * cnt], bp, cnt,
+                         false, pv, h, lam, miu, DxVx, DxVy, DxVz, DyVx, DyVy, DyVz, DzVx, DzVy, DzVz, Vx, Vy, Vz, rzk,
+                         rxs, xc);
   }
 }
 
