@@ -1309,7 +1309,9 @@ int fw_init(fw_handle h) {
     if (n % 2 == 0) f.hi[ax] = f.lo[ax] + N; else f.lo[ax] = f.hi[ax] - N;
     f.cnt = 1;
     for (int d = 0; d < 3; ++d) { f.ex[d] = f.hi[d] - f.lo[d] + 1; f.cnt *= f.ex[d]; }
-    if (f.cnt >= (1L << 31)) return fail(h, FW_ERR_ARG, "CFS-PML slab with 2^31 or more cells");
+    // the stage kernel addresses a member's 24 auxiliary volumes of a face
+    // with 32-bit offsets
+    if (24 * f.cnt >= (1L << 32)) return fail(h, FW_ERR_ARG, "CFS-PML slab of 2^32/24 or more cells");
     HIPCHK(hipMalloc(&f.d, sizeof(float) * 4 * 6 * (size_t)f.cnt * h->nbatch));
   }
   if (h->curv) {
