@@ -128,6 +128,37 @@ def summarize_trace(db, ncell, bytes_per_launch_cell):
               f"{ncell * bytes_per_launch_cell / (mean * 1e-6) / 1e9:.0f} GB/s algorithmic")
 
 
+def ae_sample(iters=100, batch=16, nblocks=2048):
+    """The SGT autoencoder's kernels alone: Adam training on random batches
+    of scaled 6 x 16^3 blocks (synthetic, uniform in [-1, 1]) and inference
+    of nblocks blocks; wall times and the convolutions' FLOPs."""
+    import numpy as np
+    from fwant_amd import ae
+    from fwant_amd._lib import DeviceArray, device_sync
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (nblocks, 6, 16, 16, 16)).astype(np.float32)
+    d = DeviceArray.from_host(x)
+    net = ae.AE(seed=5)
+    ae.train(net, d, nblocks, iters=2, batch=batch)
+    device_sync()
+    t = time.perf_counter()
+    hist = ae.train(net, d, nblocks, iters=iters, batch=batch, lr=2e-3)
+    device_sync()
+    t_train = time.perf_counter() - t
+    t = time.perf_counter()
+    ae.reconstruct(net, d, nblocks)
+    device_sync()
+    t_inf = time.perf_counter() - t
+    # 2 flops per multiply-add of a layer: conv (Co Do^3) x (Ci K^3); a
+    # transposed layer the same with input and output swapped
+    fwd = sum(2.0 * (co * do ** 3 * ci * k ** 3 if kind == "conv" else ci * di ** 3 * co * k ** 3)
+              for kind, ci, co, di, do, k, s, p_, a in ae.LAYERS)
+    return {"workload": f"autoencoder {iters} training iterations of batch {batch} (synthetic blocks), "
+                        f"inference of {nblocks} blocks", "train_s": t_train, "ms_per_iteration": 1e3 * t_train / iters,
+            "inference_s": t_inf, "final_loss": hist[-1], "forward_gflop_per_block": fwd / 1e9,
+            "train_gflop_per_iteration": 3 * fwd * batch / 1e9}
+
+
 def cpu_model():
     """CPU model string of the host (first 'model name' of /proc/cpuinfo)."""
     try:
@@ -277,6 +308,8 @@ def main():
                     help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
     ap.add_argument("--pipeline-only", action="store_true",
                     help="run only the kernels/hour pipeline sample and print its JSON (for profiling it alone)")
+    ap.add_argument("--ae-sample", action="store_true",
+                    help="run only the SGT autoencoder's training and inference sample and print its JSON")
     args = ap.parse_args()
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
@@ -296,6 +329,9 @@ def main():
         return
     if args.pipeline_only:
         print(json.dumps(pipeline_sample()), flush=True)
+        return
+    if args.ae_sample:
+        print(json.dumps(ae_sample()), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
