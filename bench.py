@@ -159,6 +159,22 @@ def ae_sample(iters=100, batch=16, nblocks=2048):
             "train_gflop_per_iteration": 3 * fwd * batch / 1e9}
 
 
+def stream_probe(gib=2, reps=10):
+    """Attainable HBM bandwidth on this GPU (fw_hbm_probe): grid-stride
+    streaming of two 2 GiB buffers with 16-byte non-temporal accesses,
+    read-only and copy (read + write 1:1), over a range of grid sizes."""
+    import ctypes as C
+    from fwant_amd._lib import lib, kcheck
+    out = {"bytes": gib << 30, "reps": reps, "runs": []}
+    for blocks in (1024, 2048, 4096, 8192, 16384):
+        r, c = C.c_float(), C.c_float()
+        kcheck(lib().fw_hbm_probe(gib << 30, blocks, reps, C.byref(r), C.byref(c)), "fw_hbm_probe")
+        out["runs"].append({"blocks": blocks, "read_gbs": r.value, "copy_gbs": c.value})
+    out["read_gbs_best"] = max(x["read_gbs"] for x in out["runs"])
+    out["copy_gbs_best"] = max(x["copy_gbs"] for x in out["runs"])
+    return out
+
+
 def cpu_model():
     """CPU model string of the host (first 'model name' of /proc/cpuinfo)."""
     try:
@@ -308,6 +324,8 @@ def main():
                     help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
     ap.add_argument("--pipeline-only", action="store_true",
                     help="run only the kernels/hour pipeline sample and print its JSON (for profiling it alone)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="measure the attainable HBM bandwidth (read-only sum, copy) and print its JSON")
     ap.add_argument("--ae-sample", action="store_true",
                     help="run only the SGT autoencoder's training and inference sample and print its JSON")
     args = ap.parse_args()
@@ -329,6 +347,9 @@ def main():
         return
     if args.pipeline_only:
         print(json.dumps(pipeline_sample()), flush=True)
+        return
+    if args.stream_probe:
+        print(json.dumps(stream_probe()), flush=True)
         return
     if args.ae_sample:
         print(json.dumps(ae_sample()), flush=True)

@@ -29,7 +29,7 @@ SYMBOLS = (
     "fwk_filter", "fwk_scatter2kernel", "fwk_scatter2kernel_noalloc", "fwk_rotate",
     "fwk_last_error", "fwae_conv_fwd", "fwae_conv_bwd_data", "fwae_conv_bwd_weight", "fwae_act_fwd", "fwae_act_bwd",
     "fwae_mse", "fwae_adam", "fwae_blocks", "fwk_assemble", "fwl_create", "fwl_solve", "fwl_destroy", "fwl_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
-    "fw_memcpy_d2d", "fw_device_sync",
+    "fw_memcpy_d2d", "fw_device_sync", "fw_hbm_probe",
 )
 
 
@@ -135,6 +135,7 @@ def _sig(lib):
         "fw_memcpy_d2h": (C.c_int, [vp, vp, C.c_size_t]),
         "fw_memcpy_d2d": (C.c_int, [vp, vp, C.c_size_t]),
         "fw_device_sync": (C.c_int, []),
+        "fw_hbm_probe": (C.c_int, [C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     }
     for name, (res, args) in S.items():
         f = getattr(lib, name)

@@ -500,6 +500,61 @@ int fw_device_sync(void) {
   return FW_OK;
 }
 
+// Attainable HBM bandwidth: grid-stride streaming over 16-byte vectors with
+// non-temporal accesses, read-only (a sum kept only to hold the loads) and
+// copy (read + write, 1:1)
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_probe_copy(const v4f_t* __restrict__ a, v4f_t* __restrict__ b, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+__global__ __launch_bounds__(256) void k_probe_read(const v4f_t* __restrict__ a, float* __restrict__ out, long n) {
+  float s = 0.f;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const v4f_t v = __builtin_nontemporal_load(a + i);
+    s += v.x + v.y + v.z + v.w;
+  }
+  if (s == -1.2345e37f) out[0] = s;  // never true for the zero-filled buffer
+}
+int fw_hbm_probe(size_t bytes, int blocks, int reps, float* read_gbs, float* copy_gbs) {
+  if (bytes < (1u << 20) || blocks <= 0 || reps <= 0 || !read_gbs || !copy_gbs)
+    return kfail(FW_ERR_ARG, "fw_hbm_probe: bad argument");
+  const long n = (long)(bytes / 16);
+  void *a = nullptr, *b = nullptr, *o = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FW_OK;
+  auto fin = [&]() {
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(a); hipFree(b); hipFree(o);
+    return rc;
+  };
+  if (hipMalloc(&a, 16 * (size_t)n) != hipSuccess || hipMalloc(&b, 16 * (size_t)n) != hipSuccess ||
+      hipMalloc(&o, 64) != hipSuccess || hipMemset(a, 0, 16 * (size_t)n) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = kfail(FW_ERR_NOMEM, "fw_hbm_probe: device buffers");
+    return fin();
+  }
+  float ms[2] = {0.f, 0.f};
+  for (int which = 0; which < 2; ++which) {
+    for (int r = -1; r < reps; ++r) {  // one untimed launch first
+      if (r == 0) hipEventRecord(e0, nullptr);
+      if (which == 0)
+        hipLaunchKernelGGL(k_probe_read, dim3(blocks), dim3(256), 0, nullptr, (const v4f_t*)a, (float*)o, n);
+      else
+        hipLaunchKernelGGL(k_probe_copy, dim3(blocks), dim3(256), 0, nullptr, (const v4f_t*)a, (v4f_t*)b, n);
+    }
+    hipEventRecord(e1, nullptr);
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms[which], e0, e1) != hipSuccess) {
+      rc = kfail(FW_ERR_HIP, "fw_hbm_probe: timing failed");
+      return fin();
+    }
+  }
+  *read_gbs = (float)(16.0 * n * reps / (ms[0] * 1e-3) / 1e9);
+  *copy_gbs = (float)(2.0 * 16.0 * n * reps / (ms[1] * 1e-3) / 1e9);
+  return fin();
+}
+
 int fwk_stress2strain(float* E, long cs, long ts, long npt, int knt, const float* lam, const float* mu, void* stream) {
   if (!E || !lam || !mu || npt <= 0 || knt <= 0) return kfail(FW_ERR_ARG, "fwk_stress2strain: bad argument");
   const unsigned gy = (unsigned)std::min(knt, 64);

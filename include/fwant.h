@@ -358,6 +358,13 @@ int fw_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int fw_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int fw_device_sync(void);
 
+/* Attainable HBM bandwidth of this GPU (no counterpart in the reference; a
+ * measurement aid for the roofline): `reps` timed launches of `blocks`
+ * 256-thread blocks streaming two device buffers of `bytes` each with
+ * 16-byte non-temporal accesses, read-only and copy (read + write 1:1);
+ * GB/s of bytes moved.  Allocates and frees its buffers. */
+int fw_hbm_probe(size_t bytes, int blocks, int reps, float *read_gbs, float *copy_gbs);
+
 #ifdef __cplusplus
 }
 #endif
