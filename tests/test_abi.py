@@ -43,3 +43,14 @@ def test_reference_step_count():
         from fwant_amd._lib import lib
         L = lib()
         assert [L.fw_reference_steps(n) for n in (0, 7, 8, 1000, 3000)] == [8, 8, 16, 1008, 3008]
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libfwant.so not built")
+def test_hbm_probe_rejects_bad_arguments():
+    # argument checks come before any device call
+    lib = C.CDLL(LIB)
+    lib.fw_hbm_probe.argtypes = [C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    r, c = C.c_float(), C.c_float()
+    assert lib.fw_hbm_probe(1024, 1024, 2, C.byref(r), C.byref(c)) != 0  # below 1 MiB
+    assert lib.fw_hbm_probe(1 << 26, 0, 2, C.byref(r), C.byref(c)) != 0
+    assert lib.fw_hbm_probe(1 << 26, 1024, 2, None, C.byref(c)) != 0

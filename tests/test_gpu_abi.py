@@ -50,3 +50,14 @@ def test_fw_stage_order_and_equivalence(gpu_lib):
     np.testing.assert_array_equal(b.receivers(), ra)
     a.close()
     b.close()
+
+
+def test_hbm_probe(gpu_lib):
+    """The bandwidth probe returns rates between zero and the 8 TB/s peak
+    (with margin for timer granularity on a small buffer)."""
+    import ctypes as C
+    from fwant_amd._lib import lib
+    r, c = C.c_float(), C.c_float()
+    assert lib().fw_hbm_probe(1 << 28, 1024, 3, C.byref(r), C.byref(c)) == 0
+    print(f"read {r.value:.0f} GB/s, copy {c.value:.0f} GB/s")
+    assert 100 < r.value < 9000 and 100 < c.value < 9000
