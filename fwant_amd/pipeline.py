@@ -120,19 +120,27 @@ def batched(model, stations, nsteps, pairs, fmt="bf16", tinv=10, step=2, V=None,
     t0 = time.perf_counter()
     dl, dm = _media_on(model, snap)
     npts = int(np.prod(snap["subc"]))
-    dec = {}
-    for q in sorted({x for p in pairs for x in p}):
+
+    def decoded(q):
+        # a station's FP32 SGT, decoded for one pair and freed after it: at
+        # the configured record lengths the FP32 SGT of every station at once
+        # would not fit in HBM beside the 16-bit store
         s.select(q)
         pc, ps, nrec, _, _ = s.snapshot_packed_device(0)
         y = DeviceArray((nrec, 6, npts), np.float32)
         kernel.sgt_decode(fmt, pc, ps, snap["subc"], nrec * 6, y)
-        dec[q] = (y, nrec)
+        return y, nrec
+
     ks = []
     for a, bb in pairs:
-        knt = min(dec[a][1], dec[bb][1])
-        RA = kernel.snapshot_series(dec[a][0].ptr, knt, 6, npts)
-        SB = kernel.snapshot_series(dec[bb][0].ptr, knt, 6, npts)
+        (ya, na), (yb, nb) = decoded(a), decoded(bb)
+        knt = min(na, nb)
+        RA = kernel.snapshot_series(ya.ptr, knt, 6, npts)
+        SB = kernel.snapshot_series(yb.ptr, knt, 6, npts)
         ks.append(_pair(RA, SB, dl, dm, model.stept * tinv, V[:knt], U[:knt], win, M0, filt))
+        device_sync()
+        ya.free()
+        yb.free()
     device_sync()
     t_kernel = time.perf_counter() - t0
     s.close()
