@@ -159,6 +159,34 @@ def ae_sample(iters=100, batch=16, nblocks=2048):
             "train_gflop_per_iteration": 3 * fwd * batch / 1e9}
 
 
+def scatter_sample(knt=400, npt=262144, reps=5):
+    """The station-pair convolution alone (fwk_scatter_param, alpha/beta):
+    6-component strain series of knt samples at npt points (seeded normal
+    values), the O(knt^2) sums per point; wall time per call and (n, m)
+    pairs per second."""
+    import numpy as np
+    from fwant_amd import kernel
+    from fwant_amd._lib import DeviceArray, device_sync
+    rng = np.random.default_rng(0)
+    R = DeviceArray.from_host(rng.standard_normal((6, knt, npt)).astype(np.float32))
+    S = DeviceArray.from_host(rng.standard_normal((6, knt, npt)).astype(np.float32))
+    lam = DeviceArray.from_host(np.full(npt, 3e10, np.float32))
+    mu = DeviceArray.from_host(np.full(npt, 3e10, np.float32))
+    A, B = kernel.sgt2scatter(R, S, lam, mu, 0.1)
+    device_sync()
+    A.free()
+    B.free()
+    t = time.perf_counter()
+    for _ in range(reps):
+        A, B = kernel.sgt2scatter(R, S, lam, mu, 0.1)
+        device_sync()
+        A.free()
+        B.free()
+    t = (time.perf_counter() - t) / reps
+    return {"workload": f"fwk_scatter_param: {npt} points, knt {knt}, 6 components", "ms_per_call": 1e3 * t,
+            "pairs_per_s": npt * knt * (knt - 1) / 2 / t}
+
+
 def stream_probe(gib=2, reps=10):
     """Attainable HBM bandwidth on this GPU (fw_hbm_probe): grid-stride
     streaming of two 2 GiB buffers with 16-byte non-temporal accesses,
@@ -326,6 +354,8 @@ def main():
                     help="run only the kernels/hour pipeline sample and print its JSON (for profiling it alone)")
     ap.add_argument("--stream-probe", action="store_true",
                     help="measure the attainable HBM bandwidth (read-only sum, copy) and print its JSON")
+    ap.add_argument("--scatter-sample", action="store_true",
+                    help="run only the station-pair convolution sample and print its JSON")
     ap.add_argument("--ae-sample", action="store_true",
                     help="run only the SGT autoencoder's training and inference sample and print its JSON")
     args = ap.parse_args()
@@ -350,6 +380,9 @@ def main():
         return
     if args.stream_probe:
         print(json.dumps(stream_probe()), flush=True)
+        return
+    if args.scatter_sample:
+        print(json.dumps(scatter_sample()), flush=True)
         return
     if args.ae_sample:
         print(json.dumps(ae_sample()), flush=True)
