@@ -44,3 +44,38 @@ def test_lsqr(gpu_lib):
         np.testing.assert_allclose(err, eo, rtol=1e-4)
     want = np.linalg.lstsq(A, b, rcond=None)[0]
     np.testing.assert_allclose(x, want, rtol=1e-8, atol=1e-10)
+
+
+@pytest.mark.parametrize("maxit", [1, 5, 16, 17, 40])
+def test_lsqr_iteration_limit(gpu_lib, maxit):
+    """The device-side stop at itct == maxit (the host reads the stop flag
+    every 16 iterations): the GPU runs exactly as many iterations as the
+    oracle, for limits below, at and across that polling period, and the
+    iterates agree to FP64 summation order."""
+    from fwant_amd import inversion
+    from oracle.pyoracle import Lib
+    import importlib
+    t = importlib.import_module("test_oracle_inversion")
+    s, A, b = t.random_system(7, nd=300, nb=120)
+    ptr, col, val, rhs = s.arrays()
+    x, err, it, res = inversion.lsqr(s, atol=1e-15, btol=1e-15, maxit=maxit)
+    xo, eo, ro = np.zeros(s.ncol), np.zeros(s.ncol), np.zeros(1)
+    ito = Lib(64).lsqr(len(rhs), s.ncol, ptr.ctypes.data, col.ctypes.data, val.ctypes.data, rhs.ctypes.data, 1e-15,
+                       1e-15, maxit, xo.ctypes.data, eo.ctypes.data, ro.ctypes.data)
+    print(f"maxit {maxit}: GPU {it}, oracle {ito} iterations; rel diff {np.linalg.norm(x - xo) / np.linalg.norm(xo):.2e}")
+    assert it == ito == maxit
+    # FP64 sums in another order: the iterates drift apart slowly (1.1e-12
+    # after 40 iterations)
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_lsqr_zero_rhs(gpu_lib):
+    """b = 0: beta = alpha = 0, so rho = 0 ends the solve before the first
+    iteration with x = 0 (the oracle does the same)."""
+    from fwant_amd import inversion
+    import importlib
+    t = importlib.import_module("test_oracle_inversion")
+    s, A, b = t.random_system(3, nd=60, nb=30)
+    s.rhs = [0.0] * len(s.rhs)
+    x, err, it, res = inversion.lsqr(s)
+    assert it == 0 and np.all(x == 0.0) and res == 0.0
