@@ -25,10 +25,12 @@ def rel(a, b):
 # (K, S, P, Di, N, Ci, Co): the network's two layer kinds at small and full
 # size, more output channels than one 16-row matrix tile, an odd volume with
 # stride 2 and no padding; the three launch shapes (waves splitting the
-# reduction of few columns, 16-column and 64-column wave tiles)
+# reduction of few columns, 16-column and 64-column wave tiles); reductions
+# longer than one LDS table pass of 1024 entries (forward 1280, backward
+# data 2560)
 @pytest.mark.parametrize("K,S,P,Di,N,Ci,Co", [(3, 1, 1, 8, 3, 5, 7), (4, 2, 1, 8, 3, 5, 7), (4, 2, 1, 16, 3, 16, 16),
                                              (3, 1, 1, 6, 2, 6, 20), (3, 2, 0, 7, 2, 4, 3), (3, 1, 1, 16, 16, 16, 6),
-                                             (3, 1, 1, 16, 64, 6, 16)])
+                                             (3, 1, 1, 16, 64, 6, 16), (4, 2, 1, 8, 2, 20, 5), (4, 1, 1, 6, 2, 3, 40)])
 def test_conv_primitives(gpu_lib, K, S, P, Di, N, Ci, Co):
     from fwant_amd._lib import DeviceArray, lib
     from oracle import ae_ref
