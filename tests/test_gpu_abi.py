@@ -53,11 +53,12 @@ def test_fw_stage_order_and_equivalence(gpu_lib):
 
 
 def test_hbm_probe(gpu_lib):
-    """The bandwidth probe returns rates between zero and the 8 TB/s peak
-    (with margin for timer granularity on a small buffer)."""
+    """The bandwidth probe returns positive, finite rates (a 256 MiB buffer
+    fits the 256 MB Infinity Cache, so the rate may exceed the HBM peak;
+    bench.py --stream-probe uses 2 GiB)."""
     import ctypes as C
     from fwant_amd._lib import lib
     r, c = C.c_float(), C.c_float()
     assert lib().fw_hbm_probe(1 << 28, 1024, 3, C.byref(r), C.byref(c)) == 0
     print(f"read {r.value:.0f} GB/s, copy {c.value:.0f} GB/s")
-    assert 100 < r.value < 9000 and 100 < c.value < 9000
+    assert 100 < r.value < 1e5 and 100 < c.value < 1e5
