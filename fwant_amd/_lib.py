@@ -27,7 +27,8 @@ SYMBOLS = (
     "fw_set_wavefield", "fw_enable_timing", "fw_get_timing",
     "fwk_stress2strain", "fwk_scatter", "fwk_scatter_param", "fwk_vscatter", "fwk_eoshift", "fwk_reparam",
     "fwk_filter", "fwk_scatter2kernel", "fwk_scatter2kernel_noalloc", "fwk_rotate",
-    "fwk_last_error", "fwae_conv_fwd", "fwae_conv_bwd_data", "fwae_conv_bwd_weight", "fwae_act_fwd", "fwae_act_bwd",
+    "fwk_last_error", "fwae_conv_fwd", "fwae_conv_bwd_data", "fwae_conv_bwd_weight", "fwae_conv_fwd_act",
+    "fwae_conv_bwd_data_act", "fwae_act_fwd", "fwae_act_bwd",
     "fwae_mse", "fwae_adam", "fwae_blocks", "fwk_assemble", "fwl_create", "fwl_solve", "fwl_destroy", "fwl_last_error", "fw_device_alloc", "fw_device_free", "fw_memcpy_h2d", "fw_memcpy_d2h",
     "fw_memcpy_d2d", "fw_device_sync", "fw_hbm_probe",
 )
@@ -118,6 +119,8 @@ def _sig(lib):
         "fwae_conv_fwd": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, vp, vp]),
         "fwae_conv_bwd_data": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, vp, vp]),
         "fwae_conv_bwd_weight": (C.c_int, [C.c_int] * 8 + [vp, vp, C.c_int, vp, vp, vp]),
+        "fwae_conv_fwd_act": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, C.c_float, vp, vp]),
+        "fwae_conv_bwd_data_act": (C.c_int, [C.c_int] * 8 + [vp, vp, vp, C.c_float, vp, vp]),
         "fwae_act_fwd": (C.c_int, [C.c_long, C.c_float, vp, vp]),
         "fwae_act_bwd": (C.c_int, [C.c_long, C.c_float, vp, vp, vp]),
         "fwae_mse": (C.c_int, [C.c_int, C.c_long, vp, vp, vp, vp, vp]),

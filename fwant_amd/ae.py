@@ -87,14 +87,15 @@ class AE:
 
     # -- forward ----------------------------------------------------------------
     def _layer_fwd(self, q, x, n, y):
+        # the activation (leaky ReLU, BF16 rounding) is applied as the
+        # convolution stores its output
         kind, ci, co, di, do, k, s, p, a = self.layers[q]
         if kind == "conv":
-            kcheck(self.L.fwae_conv_fwd(n, ci, co, di, do, k, s, p, _p(x), _p(self.w[q]), _p(self.b[q]), _p(y), None),
-                   "fwae_conv_fwd")
+            kcheck(self.L.fwae_conv_fwd_act(n, ci, co, di, do, k, s, p, _p(x), _p(self.w[q]), _p(self.b[q]), a, _p(y),
+                                            None), "fwae_conv_fwd_act")
         else:  # transposed: the conv's backward data with the roles of the channels swapped
-            kcheck(self.L.fwae_conv_bwd_data(n, co, ci, do, di, k, s, p, _p(x), _p(self.w[q]), _p(self.b[q]), _p(y),
-                                             None), "fwae_conv_bwd_data")
-        kcheck(self.L.fwae_act_fwd(y.numel(), a, _p(y), None), "fwae_act_fwd")
+            kcheck(self.L.fwae_conv_bwd_data_act(n, co, ci, do, di, k, s, p, _p(x), _p(self.w[q]), _p(self.b[q]), a,
+                                                 _p(y), None), "fwae_conv_bwd_data_act")
 
     def forward(self, x, n, layers=None, tag="t"):
         """x: DeviceArray of n inputs of the first layer in `layers` (default:

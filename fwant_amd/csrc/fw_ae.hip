@@ -99,10 +99,12 @@ __device__ Slice make_slice(const ConvDims& d, bool bwd, int cls) {
 // KSPLIT (few columns, long reductions): the block's waves share one column
 // tile and take every WPB-th group of UNR steps; their tiles are added in
 // wave order through LDS
+// act: the stored output is leaky ReLU (slope) of the sum, rounded to BF16
+// (the network's layer output, fused into the store)
 template <bool BWD, int NSUB, bool KSPLIT>
 __global__ __launch_bounds__(64 * WPB) void k_conv_mfma(ConvDims d, const float* __restrict__ src,
                                                         const float* __restrict__ w, const float* __restrict__ bias,
-                                                        float* __restrict__ dst) {
+                                                        float* __restrict__ dst, int act, float slope) {
   __shared__ int4 tab[TABMAX];  // per k: gather offset, A offset, packed tap offsets
   __shared__ float red[KSPLIT ? WPB - 1 : 1][NSUB][4][64];
   const Slice s = make_slice(d, BWD, blockIdx.z);
@@ -218,7 +220,10 @@ __global__ __launch_bounds__(64 * WPB) void k_conv_mfma(ConvDims d, const float*
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = m0 + 4 * (lane >> 4) + r;
-      if (m < s.M) dst[dsti[j] + m * mstr] = acc[j][r] + (bias ? bias[m] : 0.f);
+      if (m < s.M) {
+        const float v = acc[j][r] + (bias ? bias[m] : 0.f);
+        dst[dsti[j] + m * mstr] = act ? bf16_round(v > 0.f ? v : slope * v) : v;
+      }
     }
   }
 }
@@ -427,17 +432,17 @@ bool dims_ok(const ConvDims& d) {
 // reduction of one 16-column tile instead
 template <bool BWD>
 void launch_conv(const ConvDims& d, int nslice, long ncol, int M, const float* src, const float* w, const float* bias,
-                 float* dst, hipStream_t st) {
+                 float* dst, hipStream_t st, int act = 0, float slope = 1.f) {
   const unsigned gy = (unsigned)((M + MT - 1) / MT);
   if ((ncol + 64 * WPB - 1) / (64 * WPB) >= 1024)
     hipLaunchKernelGGL((k_conv_mfma<BWD, 4, false>), dim3((unsigned)((ncol + 64 * WPB - 1) / (64 * WPB)), gy, nslice),
-                       dim3(64 * WPB), 0, st, d, src, w, bias, dst);
+                       dim3(64 * WPB), 0, st, d, src, w, bias, dst, act, slope);
   else if ((ncol + 16 * WPB - 1) / (16 * WPB) >= 1024)
     hipLaunchKernelGGL((k_conv_mfma<BWD, 1, false>), dim3((unsigned)((ncol + 16 * WPB - 1) / (16 * WPB)), gy, nslice),
-                       dim3(64 * WPB), 0, st, d, src, w, bias, dst);
+                       dim3(64 * WPB), 0, st, d, src, w, bias, dst, act, slope);
   else
     hipLaunchKernelGGL((k_conv_mfma<BWD, 1, true>), dim3((unsigned)((ncol + 15) / 16), gy, nslice), dim3(64 * WPB), 0,
-                       st, d, src, w, bias, dst);
+                       st, d, src, w, bias, dst, act, slope);
 }
 
 }  // namespace
@@ -460,6 +465,25 @@ int fwae_conv_bwd_data(int N, int Ci, int Co, int Di, int Do, int K, int S, int 
   const long c = (Di + S - 1) / S;
   launch_conv<true>(d, S * S * S, (long)N * c * c * c, Ci, dy, w, b2, dx, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? FW_OK : fwk_fail(FW_ERR_HIP, "fwae_conv_bwd_data: launch failed");
+}
+
+// the same with the layer's activation fused into the store (fwae_act_fwd's
+// leaky ReLU and BF16 rounding)
+int fwae_conv_fwd_act(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float* x, const float* w,
+                      const float* b, float slope, float* y, void* stream) {
+  const ConvDims d{N, Ci, Co, Di, Do, K, S, P};
+  if (!dims_ok(d) || !x || !w || !y) return fwk_fail(FW_ERR_ARG, "fwae_conv_fwd_act: bad dimensions or argument");
+  launch_conv<false>(d, 1, (long)N * Do * Do * Do, Co, x, w, b, y, (hipStream_t)stream, 1, slope);
+  return hipGetLastError() == hipSuccess ? FW_OK : fwk_fail(FW_ERR_HIP, "fwae_conv_fwd_act: launch failed");
+}
+int fwae_conv_bwd_data_act(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float* dy, const float* w,
+                           const float* b2, float slope, float* dx, void* stream) {
+  const ConvDims d{N, Ci, Co, Di, Do, K, S, P};
+  if (!dims_ok(d) || !dy || !w || !dx)
+    return fwk_fail(FW_ERR_ARG, "fwae_conv_bwd_data_act: bad dimensions or argument");
+  const long c = (Di + S - 1) / S;
+  launch_conv<true>(d, S * S * S, (long)N * c * c * c, Ci, dy, w, b2, dx, (hipStream_t)stream, 1, slope);
+  return hipGetLastError() == hipSuccess ? FW_OK : fwk_fail(FW_ERR_HIP, "fwae_conv_bwd_data_act: launch failed");
 }
 
 // dw [Co][Ci][K^3] followed by db [Co] in dwb; part: scratch of nchunk*(Co*Ci*K^3+Co) floats

@@ -315,7 +315,9 @@ const char *fwk_last_error(void);
  * fwae_conv_bwd_weight writes dw [Co][Ci][K^3] then db [Co] into dwb,
  * using part (nchunk * (Co*Ci*K^3 + Co) floats) for a two-pass reduction.
  * fwae_act_fwd applies leaky ReLU (slope; 1 = identity) and rounds the
- * stored activation to BF16.  fwae_blocks cuts snapshot records
+ * stored activation to BF16; fwae_conv_fwd_act and fwae_conv_bwd_data_act
+ * are fwae_conv_fwd / fwae_conv_bwd_data with that activation applied to
+ * their output as it is stored.  fwae_blocks cuts snapshot records
  * [nrec][nc][c2][c1][c0] into [nrec][nblk][nc][16^3] blocks scaled to
  * [-1, 1] per block and component (scale [nrec][nblk][nc]).
  * ------------------------------------------------------------------------- */
@@ -325,6 +327,10 @@ int fwae_conv_bwd_data(int N, int Ci, int Co, int Di, int Do, int K, int S, int 
                        const float *b2, float *dx, void *stream);
 int fwae_conv_bwd_weight(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float *x, const float *dy,
                          int nchunk, float *part, float *dwb, void *stream);
+int fwae_conv_fwd_act(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float *x, const float *w,
+                      const float *b, float slope, float *y, void *stream);
+int fwae_conv_bwd_data_act(int N, int Ci, int Co, int Di, int Do, int K, int S, int P, const float *dy,
+                           const float *w, const float *b2, float slope, float *dx, void *stream);
 int fwae_act_fwd(long n, float slope, float *y, void *stream);
 int fwae_act_bwd(long n, float slope, const float *y, float *dy, void *stream);
 int fwae_mse(int N, long per, const float *y, const float *t, float *grad, double *loss, void *stream);

@@ -53,6 +53,19 @@ def test_conv_primitives(gpu_lib, K, S, P, Di, N, Ci, Co):
     W = w.reshape(Co, Ci, K, K, K).astype(np.float64)
     assert rel(y.to_host(), ae_ref.conv_fwd(x.astype(np.float64), W, b.astype(np.float64), S, P)) < 1e-6
     assert rel(dx.to_host(), ae_ref.conv_bwd_data(dy.astype(np.float64), W, S, P, Di)) < 1e-6
+    # the fused activation: the plain convolution then fwae_act_fwd, bit for bit
+    for slope in (0.1, 1.0):
+        ya, yp = DeviceArray((N, Co, Do, Do, Do)), DeviceArray((N, Co, Do, Do, Do))
+        assert L.fwae_conv_fwd_act(N, Ci, Co, Di, Do, K, S, P, P_(xd), P_(wd), P_(bd), slope, P_(ya), None) == 0
+        assert L.fwae_conv_fwd(N, Ci, Co, Di, Do, K, S, P, P_(xd), P_(wd), P_(bd), P_(yp), None) == 0
+        assert L.fwae_act_fwd(yp.numel(), slope, P_(yp), None) == 0
+        np.testing.assert_array_equal(ya.to_host(), yp.to_host())
+        dxa, dxp = DeviceArray((N, Ci, Di, Di, Di)), DeviceArray((N, Ci, Di, Di, Di))
+        bi = d(rng.standard_normal(Ci).astype(np.float32))
+        assert L.fwae_conv_bwd_data_act(N, Ci, Co, Di, Do, K, S, P, P_(dyd), P_(wd), P_(bi), slope, P_(dxa), None) == 0
+        assert L.fwae_conv_bwd_data(N, Ci, Co, Di, Do, K, S, P, P_(dyd), P_(wd), P_(bi), P_(dxp), None) == 0
+        assert L.fwae_act_fwd(dxp.numel(), slope, P_(dxp), None) == 0
+        np.testing.assert_array_equal(dxa.to_host(), dxp.to_host())
     gw, gb = ae_ref.conv_bwd_weight(x.astype(np.float64), dy.astype(np.float64), S, P, K)
     h = dwb.to_host()
     # each of the 16 partial sums adds N Do^3 / 16 products in FP32: the
