@@ -219,10 +219,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def pipeline_sample():
-    """Station-pair kernels/hour on the batched-sources configuration at a
-    tenth of its record lengths: 256x256x128 curvilinear model (the gradient
-    model under 20 seeded hills, mode C), 32 seeded
+def pipeline_sample(scale=10):
+    """Station-pair kernels/hour on the batched-sources configuration at
+    1/scale of its record lengths (default a tenth): 256x256x128 curvilinear
+    model (the gradient model under 20 seeded hills, mode C), 32 seeded
     surface stations with record lengths uniform in 150-400 steps (the
     configuration's 1500-4000 over 10), 6-component stress SGT every 10
     steps on the 2x decimated grid (128x128x64), 16 seeded station pairs.
@@ -233,12 +233,12 @@ def pipeline_sample():
     import numpy as np
     from scipy import signal
     from fwant_amd import models, pipeline
-    m = models.hills_256x256x128(nt=400)
+    m = models.hills_256x256x128(nt=4000 // scale)
     rng = np.random.default_rng(11)
     w = 256 - 48
     cells = rng.choice(w * w, size=32, replace=False)
     st = [(int(3 + 24 + v % w), int(3 + 24 + v // w)) for v in cells]
-    nsteps = [int(v) for v in rng.integers(150, 401, size=32)]
+    nsteps = [int(v) * 10 // scale for v in rng.integers(150, 401, size=32)]
     pairs = []
     while len(pairs) < 16:
         a, b = (int(v) for v in rng.choice(32, size=2, replace=False))
@@ -259,9 +259,9 @@ def pipeline_sample():
     tot_b = tb["setup"] + tb["steps"] + tb["kernel"]
     plain_h, batch_h = len(pairs) / tot_p * 3600.0, len(pairs) / tot_b * 3600.0
     log(f"pipeline: plain {tp}, batched fp16 {tb}, kernel rel-L2 max {max(errs):.2e} over {len(errs)} kernels")
-    return {"workload": "hills_256x256x128 curvilinear model (mode C), 32 virtual sources (record lengths 150-400 steps: the "
-                        "batched configuration's 1500-4000 over 10), 6-component stress SGT every 10 steps on the "
-                        "2x decimated grid (128x128x64), 16 station pairs",
+    return {"workload": f"hills_256x256x128 curvilinear model (mode C), 32 virtual sources (record lengths "
+                        f"{min(nsteps)}-{max(nsteps)} steps: the batched configuration's 1500-4000 over {scale}), "
+                        "6-component stress SGT every 10 steps on the 2x decimated grid (128x128x64), 16 station pairs",
             "unit": "kernels/hour",
             "plain": dict(value=plain_h, sgt="FP32 via host memory", **tp),
             "batched_fp16": dict(value=batch_h, sgt="FP16 codes with per-16^3-block FP32 scales in HBM", **tb),
@@ -352,6 +352,8 @@ def main():
                     help="run only a plain 1 GiB device copy (4 times) to calibrate the HBM counters")
     ap.add_argument("--pipeline-only", action="store_true",
                     help="run only the kernels/hour pipeline sample and print its JSON (for profiling it alone)")
+    ap.add_argument("--pipeline-scale", type=int, default=10,
+                    help="record lengths of the pipeline sample: the configured 1500-4000 steps over this")
     ap.add_argument("--stream-probe", action="store_true",
                     help="measure the attainable HBM bandwidth (read-only sum, copy) and print its JSON")
     ap.add_argument("--scatter-sample", action="store_true",
@@ -376,7 +378,7 @@ def main():
         log("copy calibration: 4 device copies of 1 GiB (read 1 GiB + write 1 GiB each)")
         return
     if args.pipeline_only:
-        print(json.dumps(pipeline_sample()), flush=True)
+        print(json.dumps(pipeline_sample(args.pipeline_scale)), flush=True)
         return
     if args.stream_probe:
         print(json.dumps(stream_probe()), flush=True)
